@@ -1,0 +1,2 @@
+"""MI355X-native backend for GridapMHD's H1-HDiv assembly + block-preconditioned Newton-Krylov hot path."""
+from . import meshers, fespaces  # noqa: F401

@@ -1,0 +1,300 @@
+"""CPU FP64 oracle for the H1-HDiv hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py
+may import this module, and only as the checker.  The product path never
+imports, links or calls anything under oracle/.
+
+Layers restated here (reference files read as text):
+  * element integrands: oracle/mhd_oracle.c (src/weakforms.jl:211-312, 670-733)
+  * Gridap ``SparseMatrixAssembler`` semantics (src/main.jl:218-224): the CSC
+    pattern is the union over cells of every (row, col) of every touched field
+    block (uu, up, uj, pu, jj, jphi, ju, phij), Dirichlet rows/cols dropped,
+    duplicates summed, rows sorted inside columns, explicit zeros kept.
+  * Newton with a direct solve per step: ``_solver(::Val{:julia})``
+    (src/main.jl:181-186) = ``NewtonSolver(LUSolver(); maxiter, rtol=1e-6)``.
+  * FGMRES + upper block-triangular preconditioner of ``Badia2024Solver``
+    (src/Solvers/badia2024.jl:2-48).
+  * Post-processing norms of ``_hunt`` (src/Applications/hunt.jl:247-260).
+
+The direct solve uses scipy's SuperLU.  A singular Jacobian (constant pressure
+mode of a fully Dirichlet velocity; constant potential when every wall is
+insulating) is made regular by pinning one dof of each null field; the pinned
+equation is implied by the others, so every other unknown is unchanged and the
+pinned field is compared modulo its constant.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Dict, Optional
+
+import numpy as np
+import scipy.sparse as sps
+import scipy.sparse.linalg as spla
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "liborc.so")
+NDOF = 129
+OFF_P, OFF_J, OFF_PHI = 81, 85, 121
+TYPES = np.array([0] * 81 + [1] * 4 + [2] * 36 + [3] * 8)
+TOUCHED = np.array([[1, 1, 1, 0], [1, 0, 0, 0], [1, 0, 1, 1], [0, 0, 1, 0]], dtype=bool)
+
+
+def build() -> str:
+    subprocess.check_call(["make", "-s", "-C", _HERE], stdout=subprocess.DEVNULL)
+    return _LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = ctypes.CDLL(_LIB)
+    return _lib
+
+
+class OrcProblem(ctypes.Structure):
+    _fields_ = [
+        ("ncells", ctypes.c_int32), ("nfree", ctypes.c_int32), ("ndir", ctypes.c_int32),
+        ("convection", ctypes.c_int32),
+        ("cell_xyz", ctypes.c_void_p), ("cell_dofs", ctypes.c_void_p), ("rt_sign", ctypes.c_void_p),
+        ("dir_vals", ctypes.c_void_p),
+        ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("sigma", ctypes.c_double), ("B", ctypes.c_double * 3),
+        ("f0", ctypes.c_double * 3), ("fx", ctypes.c_double * 9),
+        ("g0", ctypes.c_double * 3), ("gx", ctypes.c_double * 9),
+        ("zeta_u", ctypes.c_double), ("zeta_j", ctypes.c_double),
+        ("nfaces", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("face_cell", ctypes.c_void_p), ("face_lface", ctypes.c_void_p),
+        ("face_tau", ctypes.c_void_p), ("face_cw", ctypes.c_void_p), ("face_jw", ctypes.c_void_p),
+    ]
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p).value
+
+
+CONVECTION = {"none": 0, "picard": 1, "newton": 2}
+
+
+class Oracle:
+    """The oracle bound to one problem: geometry, dof tables and physics.
+
+    ``problem`` is a dict with keys cell_xyz (C,8,3), cell_dofs (C,129),
+    rt_sign (C,6), dir_vals, nfree, field sizes ``nfields`` (u,p,j,phi),
+    physics (alpha, beta, gamma, sigma, B, f0, fx, g0, gx, zeta_u, zeta_j,
+    convection) and thin-wall faces (face_cell, face_lface, face_tau,
+    face_cw, face_jw)."""
+
+    def __init__(self, problem: Dict):
+        self.pb = problem
+        self.keep = []
+        c = lambda a, dt: self.keep.append(np.ascontiguousarray(a, dtype=dt)) or self.keep[-1]
+        self.cell_xyz = c(problem["cell_xyz"], np.float64)
+        self.cell_dofs = c(problem["cell_dofs"], np.int32)
+        self.rt_sign = c(problem["rt_sign"], np.int8)
+        self.dir_vals = c(problem.get("dir_vals", np.zeros(1)), np.float64)
+        if self.dir_vals.size == 0:
+            self.dir_vals = c(np.zeros(1), np.float64)
+        self.ncells = self.cell_dofs.shape[0]
+        self.nfree = int(problem["nfree"])
+        fc = c(problem.get("face_cell", np.zeros(0)), np.int32)
+        fl = c(problem.get("face_lface", np.zeros(0)), np.int8)
+        ft = c(problem.get("face_tau", np.zeros(0)), np.float64)
+        fcw = c(problem.get("face_cw", np.zeros(0)), np.float64)
+        fjw = c(problem.get("face_jw", np.zeros(0)), np.float64)
+        self.nfaces = fc.size
+        P = OrcProblem()
+        P.ncells = self.ncells
+        P.nfree = self.nfree
+        P.ndir = self.dir_vals.size
+        P.convection = CONVECTION[problem.get("convection", "newton")]
+        P.cell_xyz, P.cell_dofs, P.rt_sign, P.dir_vals = map(_ptr, (self.cell_xyz, self.cell_dofs, self.rt_sign, self.dir_vals))
+        for k in ("alpha", "beta", "gamma", "sigma", "zeta_u", "zeta_j"):
+            setattr(P, k, float(problem.get(k, {"sigma": 1.0}.get(k, 0.0))))
+        for k, n in (("B", 3), ("f0", 3), ("fx", 9), ("g0", 3), ("gx", 9)):
+            v = np.asarray(problem.get(k, np.zeros(n)), dtype=np.float64).ravel()
+            getattr(P, k)[:] = list(v)
+        P.nfaces = self.nfaces
+        P.face_cell, P.face_lface, P.face_tau, P.face_cw, P.face_jw = map(_ptr, (fc, fl, ft, fcw, fjw))
+        self.P = P
+        L = lib()
+        for fn in ("orc_cell_residuals", "orc_cell_jacobians"):
+            getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        for fn in ("orc_face_residuals", "orc_face_jacobians"):
+            getattr(L, fn).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_cell_norms.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.orc_point_values.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        assert L.orc_struct_size() == ctypes.sizeof(OrcProblem)
+        self._pattern = None
+
+    # ---- element level ----------------------------------------------------
+    def cell_residuals(self, x, c0=0, c1=None):
+        c1 = self.ncells if c1 is None else c1
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros((c1 - c0, NDOF))
+        lib().orc_cell_residuals(ctypes.byref(self.P), _ptr(x), c0, c1, _ptr(out))
+        return out
+
+    def cell_jacobians(self, x, c0=0, c1=None):
+        c1 = self.ncells if c1 is None else c1
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros((c1 - c0, NDOF, NDOF))
+        lib().orc_cell_jacobians(ctypes.byref(self.P), _ptr(x), c0, c1, _ptr(out))
+        return out
+
+    def face_residuals(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros((self.nfaces, 36))
+        if self.nfaces:
+            lib().orc_face_residuals(ctypes.byref(self.P), _ptr(x), _ptr(out))
+        return out
+
+    def face_jacobians(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros((self.nfaces, 36, 36))
+        if self.nfaces:
+            lib().orc_face_jacobians(ctypes.byref(self.P), _ptr(x), _ptr(out))
+        return out
+
+    def set_dirichlet(self, dir_vals):
+        self.dir_vals[:] = dir_vals
+
+    # ---- global assembly (Gridap SparseMatrixAssembler semantics) ---------
+    def residual(self, x):
+        r = np.zeros(self.nfree)
+        re = self.cell_residuals(x)
+        cd = self.cell_dofs
+        m = cd >= 0
+        np.add.at(r, cd[m], re[m])
+        if self.nfaces:
+            rf = self.face_residuals(x)
+            fd = cd[self.keep_face_cell(), OFF_J:OFF_J + 36]
+            m = fd >= 0
+            np.add.at(r, fd[m], rf[m])
+        return r
+
+    def keep_face_cell(self):
+        return np.asarray(self.pb.get("face_cell", np.zeros(0)), dtype=np.int64)
+
+    def _coo(self, x=None, values=True, chunk=256):
+        cd = self.cell_dofs.astype(np.int64)
+        mask = TOUCHED[TYPES[:, None], TYPES[None, :]]
+        rows, cols, vals = [], [], []
+        for c0 in range(0, self.ncells, chunk):
+            c1 = min(self.ncells, c0 + chunk)
+            Ke = self.cell_jacobians(x, c0, c1) if values else None
+            R = np.broadcast_to(cd[c0:c1, :, None], (c1 - c0, NDOF, NDOF))
+            C = np.broadcast_to(cd[c0:c1, None, :], (c1 - c0, NDOF, NDOF))
+            m = (R >= 0) & (C >= 0) & mask[None]
+            rows.append(R[m]); cols.append(C[m])
+            vals.append(Ke[m] if values else np.zeros(int(m.sum())))
+        if self.nfaces:
+            fc = self.keep_face_cell()
+            fd = cd[fc, OFF_J:OFF_J + 36]
+            Kf = self.face_jacobians(x) if values else None
+            R = np.broadcast_to(fd[:, :, None], (fc.size, 36, 36))
+            C = np.broadcast_to(fd[:, None, :], (fc.size, 36, 36))
+            m = (R >= 0) & (C >= 0)
+            rows.append(R[m]); cols.append(C[m])
+            vals.append(Kf[m] if values else np.zeros(int(m.sum())))
+        return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+
+    def jacobian(self, x) -> sps.csc_matrix:
+        r, c, v = self._coo(x)
+        A = sps.csc_matrix((v, (r, c)), shape=(self.nfree, self.nfree))
+        A.sum_duplicates()
+        A.has_sorted_indices = False
+        A.sort_indices()
+        return A
+
+    def pattern(self):
+        """(colptr, rowval) of the CSC, 0-based, int64."""
+        if self._pattern is None:
+            r, c, _ = self._coo(values=False)
+            key = np.unique(c * np.int64(self.nfree) + r)
+            cols = key // self.nfree
+            rows = key % self.nfree
+            colptr = np.zeros(self.nfree + 1, dtype=np.int64)
+            np.add.at(colptr, cols + 1, 1)
+            self._pattern = (np.cumsum(colptr), rows.astype(np.int64))
+        return self._pattern
+
+    def jacobian_on_pattern(self, x) -> np.ndarray:
+        """nzval in the CSC order of ``pattern()`` (explicit zeros kept)."""
+        colptr, rowval = self.pattern()
+        A = self.jacobian(x)
+        # A may drop nothing (scipy keeps explicit zeros after sum_duplicates)
+        assert A.nnz == rowval.size, (A.nnz, rowval.size)
+        assert np.array_equal(A.indptr, colptr) and np.array_equal(A.indices, rowval)
+        return A.data.copy()
+
+    # ---- norms ------------------------------------------------------------
+    def cell_norms(self, x, nq=4):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros((self.ncells, 6))
+        lib().orc_cell_norms(ctypes.byref(self.P), _ptr(x), nq, _ptr(out))
+        return out
+
+    def point_values(self, x, cells, xi):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        out = np.zeros((cells.size, 17))
+        lib().orc_point_values(ctypes.byref(self.P), _ptr(x), cells.size, _ptr(cells), _ptr(xi), _ptr(out))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Solvers
+
+def null_pins(nfields: Dict[str, int], pin_p=True, pin_phi=False):
+    """Indices of the dofs pinned to remove constant-pressure/potential modes."""
+    off = 0
+    out = []
+    for f in ("u", "p", "j", "phi"):
+        if (f == "p" and pin_p) or (f == "phi" and pin_phi):
+            out.append(off)
+        off += nfields[f]
+    return out
+
+
+def direct_solve(A: sps.csc_matrix, b: np.ndarray, pins=()) -> np.ndarray:
+    """Solve A x = b with SuperLU; rows/cols of ``pins`` replaced by identity."""
+    if pins:
+        A = A.tolil(copy=True)
+        b = b.copy()
+        for k in pins:
+            A[k, :] = 0.0
+            A[:, k] = 0.0
+            A[k, k] = 1.0
+            b[k] = 0.0
+        A = A.tocsc()
+    lu = spla.splu(A, permc_spec="COLAMD")
+    return lu.solve(b)
+
+
+def newton_direct(orc: Oracle, x0: np.ndarray, maxiter=10, rtol=1e-6, atol=1e-12, pins=(), log=None):
+    """Newton with a direct solve per step (``_solver(::Val{:julia})``).
+
+    Stops when ||R|| <= max(atol, rtol ||R0||) like GridapSolvers' NewtonSolver."""
+    x = x0.copy()
+    r = orc.residual(x)
+    r0 = np.linalg.norm(r)
+    hist = [r0]
+    for it in range(maxiter):
+        if hist[-1] <= max(atol, rtol * r0):
+            break
+        A = orc.jacobian(x)
+        dx = direct_solve(A, -r, pins)
+        x += dx
+        r = orc.residual(x)
+        hist.append(np.linalg.norm(r))
+    if log is not None:
+        log["residuals"] = hist
+    return x
