@@ -1,0 +1,228 @@
+"""ctypes binding of libmhd's C ABI (include/mhd.h).
+
+This is the Python-side counterpart of the ``ccall`` glue a Julia caller
+uses; it passes plain pointers and sizes and owns nothing on the device.
+The library is required: every entry point raises if it cannot be loaded.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmhd.so")
+
+c_int32, c_int64, c_double, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+
+class MhdMesh(ctypes.Structure):
+    _fields_ = [("ncells", c_int32), ("nfaces_tw", c_int32), ("cell_xyz", c_void_p),
+                ("tw_face_cell", c_void_p), ("tw_face_lface", c_void_p), ("tw_tau", c_void_p),
+                ("tw_cw", c_void_p), ("tw_jw", c_void_p)]
+
+
+class MhdSpaces(ctypes.Structure):
+    _fields_ = [("nfree", c_int32 * 4), ("ndir", c_int32), ("pad_", c_int32), ("cell_dofs", c_void_p),
+                ("rt_sign", c_void_p), ("dir_vals", c_void_p)]
+
+
+class MhdParams(ctypes.Structure):
+    _fields_ = [("alpha", c_double), ("beta", c_double), ("gamma", c_double), ("sigma", c_double),
+                ("B", c_double * 3), ("f0", c_double * 3), ("fx", c_double * 9), ("g0", c_double * 3),
+                ("gx", c_double * 9), ("zeta_u", c_double), ("zeta_j", c_double), ("convection", c_int32),
+                ("pad_", c_int32)]
+
+
+class MhdSolverOpts(ctypes.Structure):
+    _fields_ = [("max_newton", c_int32), ("restart", c_int32), ("max_krylov", c_int32), ("precision", c_int32),
+                ("rtol", c_double), ("atol", c_double), ("ls_rtol", c_double), ("inner_iters", c_int32),
+                ("verbose", c_int32)]
+
+
+class MhdSolveInfo(ctypes.Structure):
+    _fields_ = [("newton_iters", c_int32), ("krylov_iters", c_int32), ("converged", c_int32), ("flags", c_int32),
+                ("residual0", c_double), ("residual", c_double), ("t_residual", c_double),
+                ("t_jacobian", c_double), ("t_setup", c_double), ("t_solve", c_double),
+                ("newton_residuals", c_double * 64), ("krylov_per_step", c_int32 * 64)]
+
+
+EXPORTS = {
+    "mhd_create": (ctypes.c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(MhdMesh), ctypes.POINTER(MhdSpaces),
+                                  ctypes.POINTER(MhdParams), c_int32]),
+    "mhd_destroy": (None, [c_void_p]),
+    "mhd_last_error": (ctypes.c_char_p, [c_void_p]),
+    "mhd_set_params": (ctypes.c_int, [c_void_p, ctypes.POINTER(MhdParams)]),
+    "mhd_set_dirichlet": (ctypes.c_int, [c_void_p, c_void_p]),
+    "mhd_sizes": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
+    "mhd_pattern": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "mhd_residual": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "mhd_jacobian": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "mhd_residual_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhd_jacobian_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhd_csr_arrays": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]),
+    "mhd_spmv_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhd_newton_solve": (ctypes.c_int, [c_void_p, c_void_p, ctypes.POINTER(MhdSolverOpts),
+                                        ctypes.POINTER(MhdSolveInfo)]),
+    "mhd_default_solver_opts": (None, [ctypes.POINTER(MhdSolverOpts)]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+class MhdError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libmhd and bind every exported symbol.  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MhdError(f"libmhd not built: {path} is missing (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(c_void_p).value
+
+
+CONVECTION = {"none": 0, "picard": 1, "newton": 2}
+
+
+def make_params(phys: dict) -> MhdParams:
+    p = MhdParams()
+    p.alpha = float(phys.get("alpha", 1.0))
+    p.beta = float(phys.get("beta", 1.0))
+    p.gamma = float(phys.get("gamma", 1.0))
+    p.sigma = float(phys.get("sigma", 1.0))
+    for k, n in (("B", 3), ("f0", 3), ("fx", 9), ("g0", 3), ("gx", 9)):
+        v = np.asarray(phys.get(k, np.zeros(n)), dtype=np.float64).ravel()
+        getattr(p, k)[:] = list(v)
+    p.zeta_u = float(phys.get("zeta_u", 0.0))
+    p.zeta_j = float(phys.get("zeta_j", 0.0))
+    conv = phys.get("convection", "newton")
+    p.convection = CONVECTION[conv] if isinstance(conv, str) else int(conv)
+    return p
+
+
+class Context:
+    """One libmhd context: the operator of one problem on one device."""
+
+    def __init__(self, cell_xyz, cell_dofs, rt_sign, nfree4, dir_vals, phys, faces=None, device=0):
+        self.lib = load()
+        self._keep = []
+        k = lambda a, dt: self._keep.append(np.ascontiguousarray(a, dtype=dt)) or self._keep[-1]
+        xyz = k(cell_xyz, np.float64).reshape(-1)
+        dofs = k(cell_dofs, np.int32)
+        sign = k(rt_sign, np.int8)
+        dv = k(dir_vals if len(dir_vals) else np.zeros(1), np.float64)
+        faces = faces or {}
+        fc = k(faces.get("cell", np.zeros(0)), np.int32)
+        fl = k(faces.get("lface", np.zeros(0)), np.int8)
+        ft = k(faces.get("tau", np.zeros(0)), np.float64)
+        fcw = k(faces.get("cw", np.zeros(0)), np.float64)
+        fjw = k(faces.get("jw", np.zeros(0)), np.float64)
+        m = MhdMesh(ncells=dofs.shape[0], nfaces_tw=fc.size, cell_xyz=_ptr(xyz), tw_face_cell=_ptr(fc),
+                    tw_face_lface=_ptr(fl), tw_tau=_ptr(ft), tw_cw=_ptr(fcw), tw_jw=_ptr(fjw))
+        s = MhdSpaces()
+        s.nfree[:] = [int(v) for v in nfree4]
+        s.ndir = int(len(dir_vals))
+        s.cell_dofs, s.rt_sign, s.dir_vals = _ptr(dofs), _ptr(sign), _ptr(dv)
+        self.params = make_params(phys)
+        h = c_void_p()
+        rc = self.lib.mhd_create(ctypes.byref(h), ctypes.byref(m), ctypes.byref(s), ctypes.byref(self.params), device)
+        self.h = h
+        if rc != 0:
+            msg = self.lib.mhd_last_error(h).decode() if h.value else "allocation failed"
+            self.close()
+            raise MhdError(f"mhd_create failed ({rc}): {msg}")
+        n, nnz = c_int64(), c_int64()
+        self._check(self.lib.mhd_sizes(self.h, ctypes.byref(n), ctypes.byref(nnz)), "mhd_sizes")
+        self.n, self.nnz = n.value, nnz.value
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise MhdError(f"{what} failed ({rc}): {self.lib.mhd_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.mhd_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- host-buffer entry points ------------------------------------------
+    def pattern(self):
+        colptr = np.empty(self.n + 1, dtype=np.int64)
+        rowval = np.empty(self.nnz, dtype=np.int64)
+        self._check(self.lib.mhd_pattern(self.h, _ptr(colptr), _ptr(rowval)), "mhd_pattern")
+        return colptr, rowval
+
+    def residual(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        r = np.empty(self.n)
+        self._check(self.lib.mhd_residual(self.h, _ptr(x), _ptr(r)), "mhd_residual")
+        return r
+
+    def jacobian(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        nz = np.empty(self.nnz)
+        self._check(self.lib.mhd_jacobian(self.h, _ptr(x), _ptr(nz)), "mhd_jacobian")
+        return nz
+
+    def set_params(self, phys):
+        self.params = make_params(phys)
+        self._check(self.lib.mhd_set_params(self.h, ctypes.byref(self.params)), "mhd_set_params")
+
+    def set_dirichlet(self, dir_vals):
+        dv = np.ascontiguousarray(dir_vals, dtype=np.float64)
+        self._check(self.lib.mhd_set_dirichlet(self.h, _ptr(dv)), "mhd_set_dirichlet")
+
+    # ---- device-pointer entry points (pointers as ints) ---------------------
+    def residual_d(self, d_x: int, d_r: int, stream: int = 0):
+        return self._check(self.lib.mhd_residual_d(self.h, d_x, d_r, stream or None), "mhd_residual_d")
+
+    def jacobian_d(self, d_x: int, d_val: int, stream: int = 0):
+        return self._check(self.lib.mhd_jacobian_d(self.h, d_x, d_val, stream or None), "mhd_jacobian_d")
+
+    def spmv_d(self, d_x: int, d_y: int, stream: int = 0):
+        return self._check(self.lib.mhd_spmv_d(self.h, d_x, d_y, stream or None), "mhd_spmv_d")
+
+    def newton_solve(self, x0, opts: Optional[MhdSolverOpts] = None):
+        x = np.array(x0, dtype=np.float64, copy=True)
+        if opts is None:
+            opts = MhdSolverOpts()
+            self.lib.mhd_default_solver_opts(ctypes.byref(opts))
+        info = MhdSolveInfo()
+        rc = self._check(self.lib.mhd_newton_solve(self.h, _ptr(x), ctypes.byref(opts), ctypes.byref(info)),
+                         "mhd_newton_solve")
+        return x, info, rc
+
+
+def context_from_spaces(sp, phys, dir_vals=None, device=0) -> Context:
+    """Context for a ``fespaces.Spaces`` object (thin-wall faces concatenated)."""
+    tw = sp.thin_wall_faces
+    faces = None
+    if tw:
+        faces = dict(cell=np.concatenate([t["cell"] for t in tw]), lface=np.concatenate([t["lface"] for t in tw]),
+                     tau=np.concatenate([np.full(t["cell"].size, t["tau"]) for t in tw]),
+                     cw=np.concatenate([np.full(t["cell"].size, t["cw"]) for t in tw]),
+                     jw=np.concatenate([np.full(t["cell"].size, t["jw"]) for t in tw]))
+    nf = [sp.nfree["u"], sp.nfree["p"], sp.nfree["j"], sp.nfree["phi"]]
+    dv = sp.dir_vals if dir_vals is None else dir_vals
+    return Context(sp.mesh.cell_xyz(), sp.cell_dofs, sp.rt_sign, nf, dv, phys, faces, device)

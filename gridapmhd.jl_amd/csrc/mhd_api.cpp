@@ -1,0 +1,244 @@
+// C-ABI entry points of libmhd (declared in include/mhd.h).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+
+#include "mhd_internal.h"
+
+namespace {
+
+int copy_phys(mhd_ctx *c, const mhd_params *p) {
+  MhdPhys &ph = c->phys;
+  ph.alpha = p->alpha;
+  ph.beta = p->beta;
+  ph.gamma = p->gamma;
+  ph.sigma = p->sigma;
+  for (int i = 0; i < 3; ++i) {
+    ph.B[i] = p->B[i];
+    ph.f0[i] = p->f0[i];
+    ph.g0[i] = p->g0[i];
+  }
+  for (int i = 0; i < 9; ++i) {
+    ph.fx[i] = p->fx[i];
+    ph.gx[i] = p->gx[i];
+  }
+  ph.zeta_u = p->zeta_u;
+  ph.zeta_j = p->zeta_j;
+  if (p->convection < 0 || p->convection > 2) {
+    c->err = "convection must be 0 (none), 1 (picard) or 2 (newton)";
+    return -1;
+  }
+  ph.convection = p->convection;
+  return 0;
+}
+
+template <class T>
+int dev_upload(mhd_ctx *c, T **dst, const T *src, size_t count) {
+  size_t bytes = sizeof(T) * std::max<size_t>(count, 1);
+  MHD_HIP(hipMalloc(reinterpret_cast<void **>(dst), bytes));
+  if (src && count) MHD_HIP(hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  else MHD_HIP(hipMemset(*dst, 0, bytes));
+  return 0;
+}
+
+
+}  // namespace
+
+void mhd_solver_free(mhd_ctx *c);  // mhd_solver.hip
+
+extern "C" {
+
+const char *mhd_last_error(const mhd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, const mhd_params *params,
+               int32_t device) {
+  if (!out || !mesh || !spaces || !params) return -1;
+  *out = nullptr;
+  mhd_ctx *c = new (std::nothrow) mhd_ctx();
+  if (!c) return -4;
+  *out = c;
+  c->device = device;
+  MHD_HIP(hipSetDevice(device));
+  MHD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (mesh->ncells <= 0 || !mesh->cell_xyz || !spaces->cell_dofs) {
+    c->err = "mesh/spaces: empty or missing arrays";
+    return -1;
+  }
+  if (mesh->nfaces_tw < 0 || (mesh->nfaces_tw > 0 && (!mesh->tw_face_cell || !mesh->tw_face_lface || !mesh->tw_tau ||
+                                                      !mesh->tw_cw || !mesh->tw_jw))) {
+    c->err = "mesh: thin-wall face arrays missing";
+    return -1;
+  }
+  int rc = copy_phys(c, params);
+  if (rc) return rc;
+  c->ncells = mesh->ncells;
+  c->nfaces = mesh->nfaces_tw;
+  c->ndir = spaces->ndir;
+  int64_t n = 0;
+  for (int f = 0; f < 4; ++f) {
+    if (spaces->nfree[f] < 0) {
+      c->err = "spaces: negative field size";
+      return -1;
+    }
+    c->nfree[f] = spaces->nfree[f];
+    n += spaces->nfree[f];
+  }
+  c->n = n;
+  // validate the dof table on the host before anything reaches a kernel
+  const int64_t ntab = (int64_t)c->ncells * mhd::NDOF;
+  for (int64_t k = 0; k < ntab; ++k) {
+    const int32_t g = spaces->cell_dofs[k];
+    if (g >= n || (g < 0 && -(int64_t)g - 1 >= std::max<int64_t>(c->ndir, 0))) {
+      c->err = "spaces: cell_dofs entry out of range at " + std::to_string(k);
+      return -1;
+    }
+  }
+  for (int f = 0; f < c->nfaces; ++f) {
+    const int32_t cell = mesh->tw_face_cell[f];
+    const int lf = mesh->tw_face_lface[f];
+    if (cell < 0 || cell >= c->ncells || lf < 0 || lf > 5) {
+      c->err = "mesh: thin-wall face " + std::to_string(f) + " out of range";
+      return -1;
+    }
+  }
+  if (c->ndir > 0 && !spaces->dir_vals) {
+    c->err = "spaces: dir_vals missing";
+    return -1;
+  }
+  if ((rc = dev_upload(c, &c->d_xyz, mesh->cell_xyz, (size_t)c->ncells * 24))) return rc;
+  if ((rc = dev_upload(c, &c->d_dofs, spaces->cell_dofs, (size_t)ntab))) return rc;
+  if (spaces->rt_sign) {
+    if ((rc = dev_upload(c, &c->d_sign, spaces->rt_sign, (size_t)c->ncells * 6))) return rc;
+  }
+  if ((rc = dev_upload(c, &c->d_dir, spaces->dir_vals, (size_t)std::max(c->ndir, 0)))) return rc;
+  if ((rc = dev_upload(c, &c->d_fcell, mesh->tw_face_cell, (size_t)c->nfaces))) return rc;
+  if ((rc = dev_upload(c, &c->d_flf, mesh->tw_face_lface, (size_t)c->nfaces))) return rc;
+  if ((rc = dev_upload(c, &c->d_ftau, mesh->tw_tau, (size_t)c->nfaces))) return rc;
+  if ((rc = dev_upload(c, &c->d_fcw, mesh->tw_cw, (size_t)c->nfaces))) return rc;
+  if ((rc = dev_upload(c, &c->d_fjw, mesh->tw_jw, (size_t)c->nfaces))) return rc;
+  {
+    mhd::RefTables *T = new mhd::RefTables;
+    mhd::FaceTables *F = new mhd::FaceTables;
+    mhd::build_ref_tables(T);
+    mhd::build_face_tables(F);
+    rc = dev_upload(c, &c->d_ref, T, 1);
+    if (!rc) rc = dev_upload(c, &c->d_fref, F, 1);
+    delete T;
+    delete F;
+    if (rc) return rc;
+  }
+  // sparsity pattern (host) and cell -> nonzero rank map (device)
+  mhd::build_pattern(c->ncells, spaces->cell_dofs, n, c->h_rowptr, c->h_colind);
+  c->nnz = c->h_rowptr[n];
+  if ((rc = dev_upload(c, &c->d_rowptr, c->h_rowptr.data(), (size_t)n + 1))) return rc;
+  if ((rc = dev_upload(c, &c->d_colind, c->h_colind.data(), (size_t)c->nnz))) return rc;
+  MHD_HIP(hipMalloc(&c->d_rank, sizeof(uint16_t) * (size_t)c->ncells * mhd::NENT));
+  if ((rc = mhd_build_rank_map(c))) return rc;
+  MHD_HIP(hipMalloc(&c->d_val, sizeof(double) * std::max<int64_t>(c->nnz, 1)));
+  MHD_HIP(hipMalloc(&c->d_x, sizeof(double) * std::max<int64_t>(n, 1)));
+  MHD_HIP(hipMalloc(&c->d_r, sizeof(double) * std::max<int64_t>(n, 1)));
+  return 0;
+}
+
+void mhd_destroy(mhd_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  mhd_solver_free(c);
+  void *ptrs[] = {c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw,
+                  c->d_rowptr, c->d_colind, c->d_rank, c->d_val, c->d_tperm, c->d_ref, c->d_fref, c->d_x, c->d_r};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int mhd_set_params(mhd_ctx *c, const mhd_params *p) {
+  if (!c || !p) return -1;
+  return copy_phys(c, p);
+}
+
+int mhd_set_dirichlet(mhd_ctx *c, const double *dir_vals) {
+  if (!c) return -1;
+  if (c->ndir <= 0) return 0;
+  if (!dir_vals) {
+    c->err = "dir_vals is NULL";
+    return -1;
+  }
+  MHD_HIP(hipSetDevice(c->device));
+  MHD_HIP(hipMemcpy(c->d_dir, dir_vals, sizeof(double) * c->ndir, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int mhd_sizes(mhd_ctx *c, int64_t *nfree, int64_t *nnz) {
+  if (!c) return -1;
+  if (nfree) *nfree = c->n;
+  if (nnz) *nnz = c->nnz;
+  return 0;
+}
+
+int mhd_pattern(mhd_ctx *c, int64_t *colptr, int64_t *rowval) {
+  if (!c || !colptr || !rowval) return -1;
+  for (int64_t i = 0; i <= c->n; ++i) colptr[i] = c->h_rowptr[i] + 1;
+  for (int64_t k = 0; k < c->nnz; ++k) rowval[k] = (int64_t)c->h_colind[k] + 1;
+  return 0;
+}
+
+int mhd_csr_arrays(mhd_ctx *c, const int64_t **d_rowptr, const int32_t **d_colind) {
+  if (!c) return -1;
+  if (d_rowptr) *d_rowptr = c->d_rowptr;
+  if (d_colind) *d_colind = c->d_colind;
+  return 0;
+}
+
+int mhd_residual_d(mhd_ctx *c, const double *d_x, double *d_r, void *stream) {
+  if (!c || !d_x || !d_r) return -1;
+  MHD_HIP(hipSetDevice(c->device));
+  return mhd_launch_residual(c, d_x, d_r, stream ? (hipStream_t)stream : c->stream);
+}
+
+int mhd_jacobian_d(mhd_ctx *c, const double *d_x, double *d_val, void *stream) {
+  if (!c || !d_x || !d_val) return -1;
+  MHD_HIP(hipSetDevice(c->device));
+  return mhd_launch_jacobian(c, d_x, d_val, stream ? (hipStream_t)stream : c->stream);
+}
+
+int mhd_spmv_d(mhd_ctx *c, const double *d_x, double *d_y, void *stream) {
+  if (!c || !d_x || !d_y) return -1;
+  MHD_HIP(hipSetDevice(c->device));
+  return mhd_launch_spmv(c, c->d_val, d_x, d_y, stream ? (hipStream_t)stream : c->stream);
+}
+
+int mhd_residual(mhd_ctx *c, const double *x, double *r) {
+  if (!c || !x || !r) return -1;
+  MHD_HIP(hipSetDevice(c->device));
+  MHD_HIP(hipMemcpyAsync(c->d_x, x, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+  int rc = mhd_launch_residual(c, c->d_x, c->d_r, c->stream);
+  if (rc) return rc;
+  MHD_HIP(hipMemcpyAsync(r, c->d_r, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mhd_jacobian(mhd_ctx *c, const double *x, double *nzval) {
+  if (!c || !x || !nzval) return -1;
+  MHD_HIP(hipSetDevice(c->device));
+  MHD_HIP(hipMemcpyAsync(c->d_x, x, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+  int rc = mhd_launch_jacobian(c, c->d_x, c->d_val, c->stream);
+  if (rc) return rc;
+  if (!c->d_tperm) {
+    MHD_HIP(hipMalloc(&c->d_tperm, sizeof(int64_t) * std::max<int64_t>(c->nnz, 1)));
+    if ((rc = mhd_build_tperm(c))) return rc;
+  }
+  // CSC(A) values = CSR(A) values read through the transpose permutation
+  std::vector<double> csr(c->nnz);
+  std::vector<int64_t> tp(c->nnz);
+  MHD_HIP(hipMemcpyAsync(csr.data(), c->d_val, sizeof(double) * c->nnz, hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipMemcpyAsync(tp.data(), c->d_tperm, sizeof(int64_t) * c->nnz, hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  for (int64_t k = 0; k < c->nnz; ++k) nzval[k] = csr[tp[k]];
+  return 0;
+}
+
+}  // extern "C"

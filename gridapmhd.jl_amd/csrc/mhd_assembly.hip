@@ -1,0 +1,713 @@
+// Cell-wise assembly of the H1-HDiv residual and Newton Jacobian on CDNA4.
+//
+// Weak form (src/weakforms.jl:255-312, conv at :670):
+//   R(x)(y) = int  beta grad u : grad v + alpha v.((grad u)^T u) - p div v - gamma (j x B).v
+//                 - div u q + j.s - sigma phi div s - sigma (u x B).s - div j phi_t
+//                 - f.v - g.s  [+ zeta_u Pi_p(div u) div v + zeta_j div j div s]
+//   J(x)(dx,y) = its derivative (Picard keeps only (grad du)^T u).
+// Thin-wall faces (src/weakforms.jl:721-733, residual evaluated at x):
+//   R += int_G tau (s.n)(j.n - jw + cw n.(grad j).n),  J += int_G tau (s.n)(dj.n + cw n.(grad dj).n)
+//
+// One workgroup per cell: quadrature geometry, pushed-forward bases and the
+// state of x at the 27 Gauss points are staged in LDS; the 14913 touched
+// entries of the element matrix are spread over the workgroup in the
+// canonical row-major order of the cell's rank map, so the rank-map reads and
+// the row-local scatter into the CSR values are contiguous.
+#include "mhd_internal.h"
+
+using namespace mhd;
+
+namespace {
+
+constexpr int WG = 256;
+
+struct CellLDS {
+  double W[NQ];              // w |det J|
+  union {
+    struct {
+      double Ji[NQ][9];      // (J^-1)[l][i]
+      double Jp[NQ][9];      // J[k][l] / det J
+    };
+    struct {
+      double D[4][NU];       // AL-u: int psi_k div v_b
+      double Mi[16];         // AL-u: inverse dP1 mass
+      double cu[4];          // AL-u: M^-1 int psi div u
+    };
+  };
+  double idet[NQ];           // 1 / det J
+  double xq[NQ][3];          // physical point
+  double gphi[NQ][27][3];    // physical Q2 gradients
+  double psi[NQ][36][3];     // Piola-mapped RT basis (with face signs)
+  double dpsi[NQ][36];       // its divergence
+  double u[NQ][3];           // state of x
+  double gu[NQ][9];          // grad u [i][k] = d u_k / d x_i
+  double p[NQ], jv[NQ][3], divj[NQ], ph[NQ], divu[NQ];
+  double xl[NDOF];
+  double xyz[24];
+  double sg[6];
+  double B[3], f0[3], fx[9], g0[3], gx[9];
+  int64_t rp[NDOF];
+  int32_t g[NDOF];
+};
+
+__device__ inline void inv4(const double *m, double *inv) {
+  // adjugate / determinant of a 4x4 row-major matrix
+  inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  const double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+  const double id = 1.0 / det;
+  for (int i = 0; i < 16; ++i) inv[i] *= id;
+}
+
+// Load one cell and stage everything the integrands need.  `want_state`
+// evaluates x at the quadrature points (always needed except for a linear
+// Jacobian), `want_al` the zeta_u local projection.
+__device__ void stage_cell(CellLDS &S, int c, const double *__restrict__ xyz, const int32_t *__restrict__ dofs,
+                           const int8_t *__restrict__ sign, const double *__restrict__ dir,
+                           const double *__restrict__ x, const RefTables *__restrict__ T,
+                           const int64_t *__restrict__ rowptr, const MhdPhys &ph, bool want_al) {
+  const int tid = threadIdx.x;
+  if (tid < 24) S.xyz[tid] = xyz[(int64_t)c * 24 + tid];
+  if (tid < 6) S.sg[tid] = sign ? (double)sign[(int64_t)c * 6 + tid] : 1.0;
+  if (tid == 0) {
+    for (int i = 0; i < 3; ++i) {
+      S.B[i] = ph.B[i];
+      S.f0[i] = ph.f0[i];
+      S.g0[i] = ph.g0[i];
+    }
+    for (int i = 0; i < 9; ++i) {
+      S.fx[i] = ph.fx[i];
+      S.gx[i] = ph.gx[i];
+    }
+  }
+  if (tid < NDOF) {
+    const int32_t g = dofs[(int64_t)c * NDOF + tid];
+    S.g[tid] = g;
+    S.xl[tid] = g >= 0 ? x[g] : dir[-g - 1];
+    S.rp[tid] = (g >= 0 && rowptr) ? rowptr[g] : -1;
+  }
+  __syncthreads();
+  // geometry: one thread per quadrature point
+  if (tid < NQ) {
+    const int q = tid;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, X[3] = {0, 0, 0};
+    for (int v = 0; v < 8; ++v)
+      for (int i = 0; i < 3; ++i) {
+        const double xv = S.xyz[3 * v + i];
+        X[i] += T->q1[q][v] * xv;
+        for (int l = 0; l < 3; ++l) J[i][l] += xv * T->dq1[q][v][l];
+      }
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const double id = 1.0 / det;
+    double *Ji = S.Ji[q];
+    Ji[0 * 3 + 0] = c00 * id;
+    Ji[1 * 3 + 0] = c01 * id;
+    Ji[2 * 3 + 0] = c02 * id;
+    Ji[0 * 3 + 1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[1 * 3 + 1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[2 * 3 + 1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[0 * 3 + 2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1 * 3 + 2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2 * 3 + 2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    for (int k = 0; k < 3; ++k)
+      for (int l = 0; l < 3; ++l) S.Jp[q][3 * k + l] = J[k][l] * id;
+    S.W[q] = T->w[q] * fabs(det);
+    S.idet[q] = id;
+    for (int i = 0; i < 3; ++i) S.xq[q][i] = X[i];
+  }
+  __syncthreads();
+  // push-forward of the Q2 gradients and the RT basis
+  for (int idx = tid; idx < NQ * 27; idx += WG) {
+    const int q = idx / 27, a = idx % 27;
+    const double *Ji = S.Ji[q];
+    const double d0 = T->dq2[q][a][0], d1 = T->dq2[q][a][1], d2 = T->dq2[q][a][2];
+    for (int i = 0; i < 3; ++i) S.gphi[q][a][i] = Ji[0 * 3 + i] * d0 + Ji[1 * 3 + i] * d1 + Ji[2 * 3 + i] * d2;
+  }
+  for (int idx = tid; idx < NQ * 36; idx += WG) {
+    const int q = idx / 36, r = idx % 36;
+    const double sgn = r < 24 ? S.sg[r >> 2] : 1.0;
+    const double *Jp = S.Jp[q];
+    const double r0 = T->rt[q][r][0], r1 = T->rt[q][r][1], r2 = T->rt[q][r][2];
+    for (int k = 0; k < 3; ++k) S.psi[q][r][k] = sgn * (Jp[3 * k + 0] * r0 + Jp[3 * k + 1] * r1 + Jp[3 * k + 2] * r2);
+    S.dpsi[q][r] = sgn * T->rtdiv[q][r] * S.idet[q];
+  }
+  __syncthreads();
+  // state of x at the quadrature points: 27 x 19 scalar reductions
+  for (int idx = tid; idx < NQ * 19; idx += WG) {
+    const int q = idx / 19, k = idx % 19;
+    double acc = 0.0;
+    if (k < 3) {  // u_k
+      for (int a = 0; a < 27; ++a) acc += S.xl[k * 27 + a] * T->q2[q][a];
+      S.u[q][k] = acc;
+    } else if (k < 12) {  // grad u [i][kk]
+      const int i = (k - 3) / 3, kk = (k - 3) % 3;
+      for (int a = 0; a < 27; ++a) acc += S.xl[kk * 27 + a] * S.gphi[q][a][i];
+      S.gu[q][(k - 3)] = acc;
+    } else if (k == 12) {
+      for (int m = 0; m < 4; ++m) acc += S.xl[OFF_P + m] * T->p1[q][m];
+      S.p[q] = acc;
+    } else if (k < 16) {
+      const int kk = k - 13;
+      for (int r = 0; r < 36; ++r) acc += S.xl[OFF_J + r] * S.psi[q][r][kk];
+      S.jv[q][kk] = acc;
+    } else if (k == 16) {
+      for (int r = 0; r < 36; ++r) acc += S.xl[OFF_J + r] * S.dpsi[q][r];
+      S.divj[q] = acc;
+    } else if (k == 17) {
+      for (int v = 0; v < 8; ++v) acc += S.xl[OFF_PHI + v] * T->q1[q][v];
+      S.ph[q] = acc;
+    } else {
+      for (int a = 0; a < 27; ++a)
+        for (int cc = 0; cc < 3; ++cc) acc += S.xl[cc * 27 + a] * S.gphi[q][a][cc];
+      S.divu[q] = acc;
+    }
+  }
+  if (want_al) {
+    __syncthreads();
+    for (int idx = tid; idx < 4 * NU; idx += WG) {
+      const int k = idx / NU, b = idx % NU, cb = b / 27, nb = b % 27;
+      double acc = 0.0;
+      for (int q = 0; q < NQ; ++q) acc += S.W[q] * T->p1[q][k] * S.gphi[q][nb][cb];
+      S.D[k][b] = acc;
+    }
+    if (tid == 0) {
+      double M[16];
+      double dv[4];
+      for (int k = 0; k < 4; ++k) {
+        dv[k] = 0.0;
+        for (int l = 0; l < 4; ++l) {
+          double acc = 0.0;
+          for (int q = 0; q < NQ; ++q) acc += S.W[q] * T->p1[q][k] * T->p1[q][l];
+          M[k * 4 + l] = acc;
+        }
+        for (int q = 0; q < NQ; ++q) dv[k] += S.W[q] * T->p1[q][k] * S.divu[q];
+      }
+      inv4(M, S.Mi);
+      for (int k = 0; k < 4; ++k) {
+        double acc = 0.0;
+        for (int l = 0; l < 4; ++l) acc += S.Mi[k * 4 + l] * dv[l];
+        S.cu[k] = acc;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ inline void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Decode canonical entry e -> (local row a, local column b).
+__device__ inline void decode_entry(int e, int &a, int &b) {
+  if (e < E_P) {
+    a = e / ROWS_U_LEN;
+    b = e % ROWS_U_LEN;
+  } else if (e < E_J) {
+    const int t = e - E_P;
+    a = OFF_P + t / ROWS_P_LEN;
+    b = t % ROWS_P_LEN;
+  } else if (e < E_PHI) {
+    const int t = e - E_J;
+    a = OFF_J + t / ROWS_J_LEN;
+    const int k = t % ROWS_J_LEN;
+    b = k < NU ? k : (k < NU + NJ ? OFF_J + (k - NU) : OFF_PHI + (k - NU - NJ));
+  } else {
+    const int t = e - E_PHI;
+    a = OFF_PHI + t / ROWS_PHI_LEN;
+    b = OFF_J + t % ROWS_PHI_LEN;
+  }
+}
+
+// Jacobian entry (test a, trial b) of the fluid terms.
+__device__ double jac_entry(const CellLDS &S, const RefTables *__restrict__ T, const MhdPhys &ph, int a, int b) {
+  double acc = 0.0;
+  if (a < OFF_P) {  // velocity test v = phi_na e_ca
+    const int ca = a / 27, na = a % 27;
+    if (b < OFF_P) {
+      const int cb = b / 27, nb = b % 27;
+      const bool diag = ca == cb;
+      for (int q = 0; q < NQ; ++q) {
+        double t = 0.0;
+        if (diag) {
+          const double *ga = S.gphi[q][na], *gb = S.gphi[q][nb];
+          t = ph.beta * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+          if (ph.convection)
+            t += ph.alpha * T->q2[q][na] * (S.u[q][0] * gb[0] + S.u[q][1] * gb[1] + S.u[q][2] * gb[2]);
+        }
+        if (ph.convection == 2) t += ph.alpha * T->q2[q][na] * T->q2[q][nb] * S.gu[q][3 * cb + ca];
+        acc += S.W[q] * t;
+      }
+      if (ph.zeta_u != 0.0) {
+        double al = 0.0;
+        for (int k = 0; k < 4; ++k) {
+          double m = 0.0;
+          for (int l = 0; l < 4; ++l) m += S.Mi[4 * k + l] * S.D[l][b];
+          al += S.D[k][a] * m;
+        }
+        acc += ph.zeta_u * al;
+      }
+    } else if (b < OFF_J) {  // - dp div v
+      const int k = b - OFF_P;
+      for (int q = 0; q < NQ; ++q) acc -= S.W[q] * T->p1[q][k] * S.gphi[q][na][ca];
+    } else {  // - gamma (dj x B).v
+      const int r = b - OFF_J;
+      const int i1 = ca == 2 ? 0 : ca + 1, i2 = ca == 0 ? 2 : ca - 1;
+      const double b1 = S.B[i1], b2 = S.B[i2];
+      for (int q = 0; q < NQ; ++q) {
+        const double cr = S.psi[q][r][i1] * b2 - S.psi[q][r][i2] * b1;  // (psi x B)_ca
+        acc -= ph.gamma * S.W[q] * T->q2[q][na] * cr;
+      }
+    }
+  } else if (a < OFF_J) {  // pressure test: - div du q
+    const int k = a - OFF_P;
+    const int cb = b / 27, nb = b % 27;
+    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * T->p1[q][k] * S.gphi[q][nb][cb];
+  } else if (a < OFF_PHI) {  // current test s = psi_r
+    const int r = a - OFF_J;
+    if (b < OFF_P) {  // - sigma (du x B).s
+      const int cb = b / 27, nb = b % 27;
+      const int i1 = cb == 2 ? 0 : cb + 1, i2 = cb == 0 ? 2 : cb - 1;
+      const double b1 = S.B[i1], b2 = S.B[i2];
+      for (int q = 0; q < NQ; ++q) {
+        // (e_cb x B).s = (B x s)_cb
+        const double t = b1 * S.psi[q][r][i2] - b2 * S.psi[q][r][i1];
+        acc -= ph.sigma * S.W[q] * T->q2[q][nb] * t;
+      }
+    } else if (b < OFF_PHI) {  // dj.s + zeta_j div dj div s
+      const int rb = b - OFF_J;
+      for (int q = 0; q < NQ; ++q) {
+        const double *s = S.psi[q][r], *d = S.psi[q][rb];
+        double t = s[0] * d[0] + s[1] * d[1] + s[2] * d[2];
+        if (ph.zeta_j != 0.0) t += ph.zeta_j * S.dpsi[q][r] * S.dpsi[q][rb];
+        acc += S.W[q] * t;
+      }
+    } else {  // - sigma dphi div s
+      const int v = b - OFF_PHI;
+      for (int q = 0; q < NQ; ++q) acc -= ph.sigma * S.W[q] * T->q1[q][v] * S.dpsi[q][r];
+    }
+  } else {  // potential test: - div dj phi_t
+    const int v = a - OFF_PHI, rb = b - OFF_J;
+    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * T->q1[q][v] * S.dpsi[q][rb];
+  }
+  return acc;
+}
+
+// Residual entry of test function a.
+__device__ double res_entry(const CellLDS &S, const RefTables *__restrict__ T, const MhdPhys &ph, int a) {
+  double acc = 0.0;
+  if (a < OFF_P) {
+    const int ca = a / 27, na = a % 27;
+    for (int q = 0; q < NQ; ++q) {
+      const double *gv = S.gphi[q][na];
+      const double phv = T->q2[q][na];
+      // beta grad u : grad v with v = phi e_ca -> sum_i gu[i][ca] dphi_i
+      double t = ph.beta * (S.gu[q][0 * 3 + ca] * gv[0] + S.gu[q][1 * 3 + ca] * gv[1] + S.gu[q][2 * 3 + ca] * gv[2]);
+      if (ph.convection) {
+        // ((grad u)^T u)_ca = sum_i gu[i][ca] u_i
+        const double cv = S.gu[q][0 * 3 + ca] * S.u[q][0] + S.gu[q][1 * 3 + ca] * S.u[q][1] + S.gu[q][2 * 3 + ca] * S.u[q][2];
+        t += ph.alpha * phv * cv;
+      }
+      t -= S.p[q] * gv[ca];
+      const int i1 = ca == 2 ? 0 : ca + 1, i2 = ca == 0 ? 2 : ca - 1;
+      const double jxb = S.jv[q][i1] * S.B[i2] - S.jv[q][i2] * S.B[i1];  // (j x B)_ca
+      t -= ph.gamma * jxb * phv;
+      double f = S.f0[ca];
+      for (int k = 0; k < 3; ++k) f += S.fx[3 * ca + k] * S.xq[q][k];
+      t -= f * phv;
+      acc += S.W[q] * t;
+    }
+    if (ph.zeta_u != 0.0) {
+      double al = 0.0;
+      for (int k = 0; k < 4; ++k) al += S.D[k][a] * S.cu[k];
+      acc += ph.zeta_u * al;
+    }
+  } else if (a < OFF_J) {
+    const int k = a - OFF_P;
+    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.divu[q] * T->p1[q][k];
+  } else if (a < OFF_PHI) {
+    const int r = a - OFF_J;
+    for (int q = 0; q < NQ; ++q) {
+      const double *s = S.psi[q][r];
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int i1 = i == 2 ? 0 : i + 1, i2 = i == 0 ? 2 : i - 1;
+        const double uxb = S.u[q][i1] * S.B[i2] - S.u[q][i2] * S.B[i1];
+        double g = S.g0[i];
+        for (int k = 0; k < 3; ++k) g += S.gx[3 * i + k] * S.xq[q][k];
+        t += (S.jv[q][i] - ph.sigma * uxb - g) * s[i];
+      }
+      t -= ph.sigma * S.ph[q] * S.dpsi[q][r];
+      if (ph.zeta_j != 0.0) t += ph.zeta_j * S.divj[q] * S.dpsi[q][r];
+      acc += S.W[q] * t;
+    }
+  } else {
+    const int v = a - OFF_PHI;
+    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.divj[q] * T->q1[q][v];
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(WG) void k_cell_jacobian(int ncells, const double *__restrict__ xyz,
+                                                      const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
+                                                      const double *__restrict__ dir, const double *__restrict__ x,
+                                                      const RefTables *__restrict__ T, const int64_t *__restrict__ rowptr,
+                                                      const uint16_t *__restrict__ rank, double *__restrict__ val,
+                                                      MhdPhys ph) {
+  __shared__ CellLDS S;
+  for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
+    stage_cell(S, c, xyz, dofs, sign, dir, x, T, rowptr, ph, ph.zeta_u != 0.0);
+    const uint16_t *rk = rank + (int64_t)c * NENT;
+    for (int e = threadIdx.x; e < NENT; e += WG) {
+      int a, b;
+      decode_entry(e, a, b);
+      if (S.g[a] < 0 || S.g[b] < 0) continue;
+      const double v = jac_entry(S, T, ph, a, b);
+      atomicAdd(&val[S.rp[a] + rk[e]], v);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_cell_residual(int ncells, const double *__restrict__ xyz,
+                                                      const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
+                                                      const double *__restrict__ dir, const double *__restrict__ x,
+                                                      const RefTables *__restrict__ T, double *__restrict__ r,
+                                                      MhdPhys ph) {
+  __shared__ CellLDS S;
+  for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
+    stage_cell(S, c, xyz, dofs, sign, dir, x, T, nullptr, ph, ph.zeta_u != 0.0);
+    const int a = threadIdx.x;
+    if (a < NDOF && S.g[a] >= 0) atomicAdd(&r[S.g[a]], res_entry(S, T, ph, a));
+    __syncthreads();
+  }
+}
+
+// Thin-wall face terms: one 64-wide workgroup per boundary face.
+struct FaceLDS {
+  double W[NQF];
+  double n[NQF][3];
+  double psi[NQF][36][3];
+  double gpsi[NQF][36][9];   // [i][k] = d psi_k / d x_i
+  double jn[NQF], jnn[NQF];
+  double xl[NJ];
+  double xyz[24];
+  int32_t g[NJ];
+  int64_t rp[NJ];
+};
+
+__device__ void stage_face(FaceLDS &S, int f, const double *__restrict__ xyz, const int32_t *__restrict__ dofs,
+                           const int8_t *__restrict__ sign, const double *__restrict__ dir,
+                           const double *__restrict__ x, const FaceTables *__restrict__ F,
+                           const int32_t *__restrict__ fcell, const int8_t *__restrict__ flf,
+                           const int64_t *__restrict__ rowptr, int &cell, int &lf) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  cell = fcell[f];
+  lf = flf[f];
+  if (tid < 24) S.xyz[tid] = xyz[(int64_t)cell * 24 + tid];
+  if (tid < NJ) {
+    const int32_t g = dofs[(int64_t)cell * NDOF + OFF_J + tid];
+    S.g[tid] = g;
+    S.xl[tid] = g >= 0 ? x[g] : dir[-g - 1];
+    S.rp[tid] = (g >= 0 && rowptr) ? rowptr[g] : -1;
+  }
+  __syncthreads();
+  __shared__ double Jq[NQF][9], Jiq[NQF][9], detq[NQF];
+  if (tid < NQF) {
+    const int q = tid;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int v = 0; v < 8; ++v)
+      for (int i = 0; i < 3; ++i)
+        for (int l = 0; l < 3; ++l) J[i][l] += S.xyz[3 * v + i] * F->dq1[lf][q][v][l];
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const double id = 1.0 / det;
+    double Ji[3][3];
+    Ji[0][0] = c00 * id;
+    Ji[1][0] = c01 * id;
+    Ji[2][0] = c02 * id;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    const int d = 2 - lf / 2;
+    const double nh = (lf & 1) ? 1.0 : -1.0;
+    double n[3], nn = 0.0;
+    for (int i = 0; i < 3; ++i) {
+      n[i] = Ji[d][i] * nh;
+      nn += n[i] * n[i];
+    }
+    nn = sqrt(nn);
+    for (int i = 0; i < 3; ++i) S.n[q][i] = n[i] / nn;
+    S.W[q] = F->w[q] * fabs(det) * nn;
+    for (int i = 0; i < 3; ++i)
+      for (int l = 0; l < 3; ++l) {
+        Jq[q][3 * i + l] = J[i][l];
+        Jiq[q][3 * i + l] = Ji[i][l];
+      }
+    detq[q] = det;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < NQF * 36; idx += nt) {
+    const int q = idx / 36, r = idx % 36;
+    const double sgn = r < 24 ? (sign ? (double)sign[(int64_t)cell * 6 + (r >> 2)] : 1.0) : 1.0;
+    const double id = sgn / detq[q];
+    const double *J = Jq[q], *Ji = Jiq[q];
+    const double *rt = F->rt[lf][q][r];
+    for (int k = 0; k < 3; ++k) S.psi[q][r][k] = id * (J[3 * k + 0] * rt[0] + J[3 * k + 1] * rt[1] + J[3 * k + 2] * rt[2]);
+    // grad psi [i][k] = id * sum_l J[k][l] sum_m drt[l][m] Ji[m][i]  (affine push-forward)
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) {
+        double acc = 0.0;
+        for (int l = 0; l < 3; ++l) {
+          double dl = 0.0;
+          for (int m = 0; m < 3; ++m) dl += F->drt[lf][q][r][l][m] * Ji[3 * m + i];
+          acc += J[3 * k + l] * dl;
+        }
+        S.gpsi[q][r][3 * i + k] = id * acc;
+      }
+  }
+  __syncthreads();
+  if (tid < NQF) {
+    const int q = tid;
+    const double *n = S.n[q];
+    double jn = 0.0, jnn = 0.0;
+    for (int r = 0; r < 36; ++r) {
+      const double *s = S.psi[q][r];
+      const double *gs = S.gpsi[q][r];
+      jn += S.xl[r] * (s[0] * n[0] + s[1] * n[1] + s[2] * n[2]);
+      double t = 0.0;
+      for (int i = 0; i < 3; ++i) t += n[i] * (gs[3 * i + 0] * n[0] + gs[3 * i + 1] * n[1] + gs[3 * i + 2] * n[2]);
+      jnn += S.xl[r] * t;
+    }
+    S.jn[q] = jn;
+    S.jnn[q] = jnn;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void k_face_residual(int nfaces, const double *__restrict__ xyz,
+                                                      const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
+                                                      const double *__restrict__ dir, const double *__restrict__ x,
+                                                      const FaceTables *__restrict__ F, const int32_t *__restrict__ fcell,
+                                                      const int8_t *__restrict__ flf, const double *__restrict__ ftau,
+                                                      const double *__restrict__ fcw, const double *__restrict__ fjw,
+                                                      double *__restrict__ r) {
+  __shared__ FaceLDS S;
+  for (int f = blockIdx.x; f < nfaces; f += gridDim.x) {
+    int cell, lf;
+    stage_face(S, f, xyz, dofs, sign, dir, x, F, fcell, flf, nullptr, cell, lf);
+    const int a = threadIdx.x;
+    if (a < NJ && S.g[a] >= 0) {
+      const double tau = ftau[f], cw = fcw[f], jw = fjw[f];
+      double acc = 0.0;
+      for (int q = 0; q < NQF; ++q) {
+        const double *s = S.psi[q][a], *n = S.n[q];
+        const double sn = s[0] * n[0] + s[1] * n[1] + s[2] * n[2];
+        acc += S.W[q] * tau * sn * (S.jn[q] - jw + cw * S.jnn[q]);
+      }
+      atomicAdd(&r[S.g[a]], acc);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64) void k_face_jacobian(int nfaces, const double *__restrict__ xyz,
+                                                      const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
+                                                      const double *__restrict__ dir, const double *__restrict__ x,
+                                                      const FaceTables *__restrict__ F, const int32_t *__restrict__ fcell,
+                                                      const int8_t *__restrict__ flf, const double *__restrict__ ftau,
+                                                      const double *__restrict__ fcw, const int64_t *__restrict__ rowptr,
+                                                      const uint16_t *__restrict__ rank, double *__restrict__ val) {
+  __shared__ FaceLDS S;
+  for (int f = blockIdx.x; f < nfaces; f += gridDim.x) {
+    int cell, lf;
+    stage_face(S, f, xyz, dofs, sign, dir, x, F, fcell, flf, rowptr, cell, lf);
+    const double tau = ftau[f], cw = fcw[f];
+    const uint16_t *rk = rank + (int64_t)cell * NENT;
+    for (int idx = threadIdx.x; idx < NJ * NJ; idx += blockDim.x) {
+      const int a = idx / NJ, b = idx % NJ;
+      if (S.g[a] < 0 || S.g[b] < 0) continue;
+      double acc = 0.0;
+      for (int q = 0; q < NQF; ++q) {
+        const double *n = S.n[q], *s = S.psi[q][a], *d = S.psi[q][b], *gd = S.gpsi[q][b];
+        const double sn = s[0] * n[0] + s[1] * n[1] + s[2] * n[2];
+        const double dn = d[0] * n[0] + d[1] * n[1] + d[2] * n[2];
+        double dnn = 0.0;
+        for (int i = 0; i < 3; ++i) dnn += n[i] * (gd[3 * i + 0] * n[0] + gd[3 * i + 1] * n[1] + gd[3 * i + 2] * n[2]);
+        acc += S.W[q] * tau * sn * (dn + cw * dnn);
+      }
+      // jj block entry of j row a, j column b inside the cell's rank map
+      const int e = E_J + a * ROWS_J_LEN + NU + b;
+      atomicAdd(&val[S.rp[a] + rk[e]], acc);
+    }
+    __syncthreads();
+  }
+}
+
+// Rank of every touched cell entry inside its CSR row (binary search).
+__global__ void k_rank_map(int ncells, const int32_t *__restrict__ dofs, const int64_t *__restrict__ rowptr,
+                           const int32_t *__restrict__ colind, uint16_t *__restrict__ rank, int *__restrict__ bad) {
+  const int64_t total = (int64_t)ncells * NENT;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(t / NENT), e = (int)(t % NENT);
+    int a, b;
+    decode_entry(e, a, b);
+    const int32_t ga = dofs[(int64_t)c * NDOF + a], gb = dofs[(int64_t)c * NDOF + b];
+    uint16_t out = RANK_NONE;
+    if (ga >= 0 && gb >= 0) {
+      int64_t lo = rowptr[ga], hi = rowptr[ga + 1];
+      const int64_t base = lo;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (colind[mid] < gb) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < rowptr[ga + 1] && colind[lo] == gb && lo - base < RANK_NONE) out = (uint16_t)(lo - base);
+      else atomicAdd(bad, 1);
+    }
+    rank[t] = out;
+  }
+}
+
+// tperm[p] = CSR position of the transposed entry of p.
+__global__ void k_tperm(int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                        int64_t *__restrict__ tperm, int *__restrict__ bad) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+      const int32_t j = colind[p];
+      int64_t lo = rowptr[j], hi = rowptr[j + 1];
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (colind[mid] < i) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < rowptr[j + 1] && colind[lo] == i) tperm[p] = lo;
+      else {
+        tperm[p] = -1;
+        atomicAdd(bad, 1);
+      }
+    }
+  }
+}
+
+// CSR SpMV, one 64-lane wavefront per group of rows, `LPR` lanes per row.
+template <int LPR>
+__global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ colind, const double *__restrict__ val,
+                                              const double *__restrict__ x, double *__restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gid / LPR;
+  const int lane = threadIdx.x % LPR;
+  double acc = 0.0;
+  if (row < n) {
+    const int64_t b = rowptr[row], e = rowptr[row + 1];
+    for (int64_t p = b + lane; p < e; p += LPR) acc += val[p] * x[colind[p]];
+  }
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
+  if (row < n && lane == 0) y[row] = acc;
+}
+
+}  // namespace
+
+int mhd_launch_jacobian(mhd_ctx *c, const double *d_x, double *d_val, hipStream_t s) {
+  MHD_HIP(hipMemsetAsync(d_val, 0, sizeof(double) * c->nnz, s));
+  if (c->ncells > 0) {
+    const int grid = c->ncells;
+    hipLaunchKernelGGL(k_cell_jacobian, dim3(grid), dim3(WG), 0, s, c->ncells, c->d_xyz, c->d_dofs, c->d_sign,
+                       c->d_dir, d_x, c->d_ref, c->d_rowptr, c->d_rank, d_val, c->phys);
+    MHD_HIP(hipGetLastError());
+  }
+  if (c->nfaces > 0) {
+    hipLaunchKernelGGL(k_face_jacobian, dim3(c->nfaces), dim3(64), 0, s, c->nfaces, c->d_xyz, c->d_dofs, c->d_sign,
+                       c->d_dir, d_x, c->d_fref, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_rowptr, c->d_rank,
+                       d_val);
+    MHD_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+int mhd_launch_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s) {
+  MHD_HIP(hipMemsetAsync(d_r, 0, sizeof(double) * c->n, s));
+  if (c->ncells > 0) {
+    hipLaunchKernelGGL(k_cell_residual, dim3(c->ncells), dim3(WG), 0, s, c->ncells, c->d_xyz, c->d_dofs, c->d_sign,
+                       c->d_dir, d_x, c->d_ref, d_r, c->phys);
+    MHD_HIP(hipGetLastError());
+  }
+  if (c->nfaces > 0) {
+    hipLaunchKernelGGL(k_face_residual, dim3(c->nfaces), dim3(64), 0, s, c->nfaces, c->d_xyz, c->d_dofs, c->d_sign,
+                       c->d_dir, d_x, c->d_fref, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw, d_r);
+    MHD_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+int mhd_build_rank_map(mhd_ctx *c) {
+  int *d_bad = nullptr;
+  MHD_HIP(hipMalloc(&d_bad, sizeof(int)));
+  MHD_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), c->stream));
+  const int64_t total = (int64_t)c->ncells * NENT;
+  const int grid = (int)std::min<int64_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_rank_map, dim3(grid), dim3(256), 0, c->stream, c->ncells, c->d_dofs, c->d_rowptr, c->d_colind,
+                     c->d_rank, d_bad);
+  MHD_HIP(hipGetLastError());
+  int bad = 0;
+  MHD_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  MHD_HIP(hipFree(d_bad));
+  if (bad) {
+    c->err = "rank map: " + std::to_string(bad) + " entries missing from the pattern";
+    return -3;
+  }
+  return 0;
+}
+
+int mhd_build_tperm(mhd_ctx *c) {
+  int *d_bad = nullptr;
+  MHD_HIP(hipMalloc(&d_bad, sizeof(int)));
+  MHD_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), c->stream));
+  const int grid = (int)std::min<int64_t>((c->n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_tperm, dim3(grid), dim3(256), 0, c->stream, c->n, c->d_rowptr, c->d_colind, c->d_tperm, d_bad);
+  MHD_HIP(hipGetLastError());
+  int bad = 0;
+  MHD_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  MHD_HIP(hipFree(d_bad));
+  if (bad) {
+    c->err = "pattern is not structurally symmetric (" + std::to_string(bad) + " entries)";
+    return -3;
+  }
+  return 0;
+}
+
+int mhd_launch_spmv(mhd_ctx *c, const double *d_val, const double *d_x, double *d_y, hipStream_t s) {
+  constexpr int LPR = 16;
+  const int64_t threads = c->n * LPR;
+  const int grid = (int)((threads + 255) / 256);
+  hipLaunchKernelGGL(k_spmv<LPR>, dim3(grid), dim3(256), 0, s, c->n, c->d_rowptr, c->d_colind, d_val, d_x, d_y);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}

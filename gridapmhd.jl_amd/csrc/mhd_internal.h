@@ -1,0 +1,114 @@
+// Internal declarations of libmhd (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mhd.h"
+
+namespace mhd {
+
+constexpr int NDOF = 129;
+constexpr int NU = 81, NP = 4, NJ = 36, NPHI = 8;
+constexpr int OFF_P = 81, OFF_J = 85, OFF_PHI = 121;
+constexpr int NQ = 27;          // 3x3x3 Gauss points per cell (quadrature degree 5)
+constexpr int NQF = 9;          // 3x3 Gauss points per face
+// Touched entries of one cell in the canonical row-major enumeration:
+// u rows (81) x [u p j] (121), p rows (4) x u (81), j rows (36) x [u j phi] (125), phi rows (8) x j (36).
+constexpr int ROWS_U_LEN = 121, ROWS_P_LEN = 81, ROWS_J_LEN = 125, ROWS_PHI_LEN = 36;
+constexpr int E_U = 0;
+constexpr int E_P = E_U + NU * ROWS_U_LEN;          // 9801
+constexpr int E_J = E_P + NP * ROWS_P_LEN;          // 10125
+constexpr int E_PHI = E_J + NJ * ROWS_J_LEN;        // 14625
+constexpr int NENT = E_PHI + NPHI * ROWS_PHI_LEN;   // 14913
+constexpr uint16_t RANK_NONE = 0xFFFF;
+
+// Reference tables at the cell quadrature points (double precision).
+struct RefTables {
+  double w[NQ];                 // tensor Gauss weights on [0,1]^3
+  double q2[NQ][27];            // Q2 scalar shape values
+  double dq2[NQ][27][3];        // reference gradients
+  double p1[NQ][4];             // dP1
+  double q1[NQ][8];             // dQ1 / geometry shape values
+  double dq1[NQ][8][3];         // geometry shape derivatives
+  double rt[NQ][36][3];         // RT reference field
+  double rtdiv[NQ][36];         // reference divergence
+};
+
+// Reference tables at the face quadrature points of each local face.
+struct FaceTables {
+  double w[NQF];
+  double q1[6][NQF][8];
+  double dq1[6][NQF][8][3];
+  double rt[6][NQF][36][3];
+  double drt[6][NQF][36][3][3];  // d rt_l / d xi_m  ([l][m])
+};
+
+void build_ref_tables(RefTables *T);
+void build_face_tables(FaceTables *F);
+
+// Pattern (host): CSR with int64 row pointers and int32 sorted column indices.
+void build_pattern(int32_t ncells, const int32_t *cell_dofs, int64_t n,
+                   std::vector<int64_t> &rowptr, std::vector<int32_t> &colind);
+
+}  // namespace mhd
+
+// Device-side mirror of the physics parameters.
+struct MhdPhys {
+  double alpha, beta, gamma, sigma;
+  double B[3];
+  double f0[3], fx[9], g0[3], gx[9];
+  double zeta_u, zeta_j;
+  int convection;
+};
+
+struct mhd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int32_t ncells = 0, nfaces = 0, ndir = 0;
+  int64_t n = 0, nnz = 0;
+  int32_t nfree[4] = {0, 0, 0, 0};
+  MhdPhys phys{};
+  // device arrays
+  double *d_xyz = nullptr;        // [ncells][24]
+  int32_t *d_dofs = nullptr;      // [ncells][129]
+  int8_t *d_sign = nullptr;       // [ncells][6]
+  double *d_dir = nullptr;        // [max(ndir,1)]
+  int32_t *d_fcell = nullptr;     // thin-wall faces
+  int8_t *d_flf = nullptr;
+  double *d_ftau = nullptr, *d_fcw = nullptr, *d_fjw = nullptr;
+  int64_t *d_rowptr = nullptr;    // [n+1]
+  int32_t *d_colind = nullptr;    // [nnz]
+  uint16_t *d_rank = nullptr;     // [ncells][NENT] rank of each touched entry inside its row
+  double *d_val = nullptr;        // [nnz] CSR values of the last Jacobian
+  int64_t *d_tperm = nullptr;     // [nnz] CSR position of the transposed entry (CSC export)
+  mhd::RefTables *d_ref = nullptr;
+  mhd::FaceTables *d_fref = nullptr;
+  // scratch
+  double *d_x = nullptr, *d_r = nullptr;
+  // host copies needed by the API
+  std::vector<int64_t> h_rowptr;
+  std::vector<int32_t> h_colind;
+  // solver workspace (allocated lazily)
+  struct Solver *solver = nullptr;
+};
+
+// Launchers (mhd_assembly.hip)
+int mhd_launch_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s);
+int mhd_launch_jacobian(mhd_ctx *c, const double *d_x, double *d_val, hipStream_t s);
+int mhd_build_rank_map(mhd_ctx *c);
+int mhd_build_tperm(mhd_ctx *c);
+int mhd_launch_spmv(mhd_ctx *c, const double *d_val, const double *d_x, double *d_y, hipStream_t s);
+
+#define MHD_HIP(call)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      c->err = std::string(#call) + ": " + hipGetErrorString(e_);                     \
+      return -2;                                                                      \
+    }                                                                                 \
+  } while (0)

@@ -1,0 +1,143 @@
+/*
+ * mhd.h -- C ABI of libmhd, the MI355X-native backend for GridapMHD's hot path:
+ * H1-HDiv residual / Newton-Jacobian assembly and the block-preconditioned
+ * Newton-FGMRES solver.
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success, a
+ * positive value for a solver outcome (not converged, breakdown), a negative
+ * value for an error; mhd_last_error() gives the text.  Calls are synchronous
+ * unless they take a stream.  One context is single-threaded.
+ *
+ * Host buffers are owned by the caller; the context owns all device memory.
+ * Functions with the _d suffix take device pointers (hipDeviceptr_t) and an
+ * optional hipStream_t (NULL = the context's stream); they return after the
+ * work is enqueued, not after it completes.
+ */
+#ifndef MHD_H
+#define MHD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Local dof layout of one hexahedral cell (129 dofs):
+ *   u   [0, 81)    c*27 + a, Q2 node a = i + 3j + 9k at (i/2, j/2, k/2)
+ *   p   [81, 85)   discontinuous P1: 1-x-y-z, x, y, z (reference coordinates)
+ *   j   [85, 121)  Raviart-Thomas order 1: 24 face dofs (faces z-, z+, y-, y+,
+ *                  x-, x+; 4 each at the face's 2x2 Gauss points), 12 interior
+ *   phi [121, 129) discontinuous Q1 at vertex v = i + 2j + 4k
+ * Local vertex v = i + 2j + 4k of the trilinear geometry. */
+#define MHD_NDOF_CELL 129
+
+typedef struct mhd_ctx mhd_ctx;
+
+/* Geometry and thin-wall faces.  Replaces the model / triangulations of
+ * setup_geometry! and the thin-wall measures of retrieve_bcs_params
+ * (src/geometry.jl:6-36, src/weakforms.jl:153-162). */
+typedef struct {
+  int32_t ncells;
+  int32_t nfaces_tw;            /* thin-wall boundary faces */
+  const double *cell_xyz;       /* [ncells][8][3] */
+  const int32_t *tw_face_cell;  /* [nfaces_tw] owning cell */
+  const int8_t *tw_face_lface;  /* [nfaces_tw] local face 0..5 (z-, z+, y-, y+, x-, x+) */
+  const double *tw_tau;         /* [nfaces_tw] penalty tau */
+  const double *tw_cw;          /* [nfaces_tw] wall conductance ratio cw */
+  const double *tw_jw;          /* [nfaces_tw] prescribed jw */
+} mhd_mesh;
+
+/* Cell dof tables of the multifield space.  Replaces setup_fe_spaces
+ * (src/fespaces.jl:13-46): Consecutive [u; p; j; phi] free numbering. */
+typedef struct {
+  int32_t nfree[4];             /* free dofs of u, p, j, phi */
+  int32_t ndir;                 /* Dirichlet values */
+  int32_t pad_;
+  const int32_t *cell_dofs;     /* [ncells][129]: >= 0 free id, < 0 -> Dirichlet value -(id+1) */
+  const int8_t *rt_sign;        /* [ncells][6] face orientation signs, NULL = all +1 */
+  const double *dir_vals;       /* [ndir] */
+} mhd_spaces;
+
+/* Fluid parameters.  Replaces params[:fluid] as read by retrieve_fluid_params
+ * (src/weakforms.jl:73-83; defaults src/parameters.jl:710-719). */
+typedef struct {
+  double alpha, beta, gamma, sigma;
+  double B[3];
+  double f0[3], fx[9];          /* f(x) = f0 + fx x (row-major) */
+  double g0[3], gx[9];          /* g(x) = g0 + gx x */
+  double zeta_u, zeta_j;        /* augmented-Lagrangian weights */
+  int32_t convection;           /* 0 :none, 1 :picard, 2 :newton */
+  int32_t pad_;
+} mhd_params;
+
+/* Solver options.  Replaces default_solver_params(::Val{:badia2024}) and the
+ * solver built by Badia2024Solver (src/parameters.jl:259-272,
+ * src/Solvers/badia2024.jl:2-48). */
+typedef struct {
+  int32_t max_newton;           /* :niter (20) */
+  int32_t restart;              /* FGMRES Krylov dimension m (:niter_ls, 15) */
+  int32_t max_krylov;           /* FGMRES iteration cap per Newton step */
+  int32_t precision;            /* 0 FP64, 1 mixed (FP32 inner / BF16 smoother blocks, FP64 refinement) */
+  double rtol, atol;            /* Newton: ||R|| <= max(atol, rtol ||R0||) */
+  double ls_rtol;               /* linear: ||r|| <= ls_rtol ||b|| */
+  int32_t inner_iters;          /* smoothing sweeps of the [u,j] block solve */
+  int32_t verbose;
+} mhd_solver_opts;
+
+typedef struct {
+  int32_t newton_iters;
+  int32_t krylov_iters;         /* total over the Newton steps */
+  int32_t converged;
+  int32_t flags;
+  double residual0, residual;   /* first and last nonlinear residual norm */
+  double t_residual, t_jacobian, t_setup, t_solve;  /* seconds, summed */
+  double newton_residuals[64];
+  int32_t krylov_per_step[64];
+} mhd_solve_info;
+
+/* Create a context on `device`: copies inputs to HBM, builds the CSR/CSC
+ * pattern and the cell-to-nonzero map.  Replaces _fe_operator +
+ * SparseMatrixAssembler (src/main.jl:218-224). */
+int mhd_create(mhd_ctx **ctx, const mhd_mesh *mesh, const mhd_spaces *spaces,
+               const mhd_params *params, int32_t device);
+void mhd_destroy(mhd_ctx *ctx);
+const char *mhd_last_error(const mhd_ctx *ctx);
+
+/* Update fluid parameters / Dirichlet values (e.g. continuation, adjoint FD). */
+int mhd_set_params(mhd_ctx *ctx, const mhd_params *params);
+int mhd_set_dirichlet(mhd_ctx *ctx, const double *dir_vals);
+
+/* Sizes of the system. */
+int mhd_sizes(mhd_ctx *ctx, int64_t *nfree, int64_t *nnz);
+
+/* Jacobian sparsity in Gridap's SparseMatrixCSC{Float64,Int64} layout:
+ * 1-based colptr[n+1], rowval[nnz], rows sorted in every column. */
+int mhd_pattern(mhd_ctx *ctx, int64_t *colptr, int64_t *rowval);
+
+/* residual(op, xh) (src/weakforms.jl:211-232 + thin wall 721-726) on host buffers. */
+int mhd_residual(mhd_ctx *ctx, const double *x_free, double *r_free);
+
+/* jacobian(op, xh) (src/weakforms.jl:234-253 + 728-733): nzval in CSC order of mhd_pattern. */
+int mhd_jacobian(mhd_ctx *ctx, const double *x_free, double *nzval);
+
+/* Device-pointer variants (inputs resident in HBM).  The Jacobian values are
+ * written in the library's CSR order (row-major, columns sorted); the pattern
+ * is structurally symmetric, so CSR(A) and CSC(A) share index arrays and the
+ * CSC values are CSR(A^T). */
+int mhd_residual_d(mhd_ctx *ctx, const double *d_x, double *d_r, void *stream);
+int mhd_jacobian_d(mhd_ctx *ctx, const double *d_x, double *d_nzval_csr, void *stream);
+int mhd_csr_arrays(mhd_ctx *ctx, const int64_t **d_rowptr, const int32_t **d_colind);
+
+/* y = A x with the last assembled Jacobian (device pointers). */
+int mhd_spmv_d(mhd_ctx *ctx, const double *d_x, double *d_y, void *stream);
+
+/* Newton + FGMRES + Badia2024 block preconditioner.  Replaces
+ * solve!(xh, Badia2024Solver(op, params), op) (src/main.jl:150, 275).
+ * x_free is the initial guess on entry, the solution on return. */
+int mhd_newton_solve(mhd_ctx *ctx, double *x_free, const mhd_solver_opts *opts, mhd_solve_info *info);
+void mhd_default_solver_opts(mhd_solver_opts *opts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
