@@ -20,12 +20,13 @@ c_int32, c_int64, c_double, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_
 
 class MhdMesh(ctypes.Structure):
     _fields_ = [("ncells", c_int32), ("nfaces_tw", c_int32), ("cell_xyz", c_void_p),
+                ("cell_verts", c_void_p), ("nverts", c_int32), ("pad_", c_int32),
                 ("tw_face_cell", c_void_p), ("tw_face_lface", c_void_p), ("tw_tau", c_void_p),
                 ("tw_cw", c_void_p), ("tw_jw", c_void_p)]
 
 
 class MhdSpaces(ctypes.Structure):
-    _fields_ = [("nfree", c_int32 * 4), ("ndir", c_int32), ("pad_", c_int32), ("cell_dofs", c_void_p),
+    _fields_ = [("nfree", c_int32 * 4), ("ndir", c_int32), ("block_style", c_int32), ("cell_dofs", c_void_p),
                 ("rt_sign", c_void_p), ("dir_vals", c_void_p)]
 
 
@@ -118,7 +119,8 @@ def make_params(phys: dict) -> MhdParams:
 class Context:
     """One libmhd context: the operator of one problem on one device."""
 
-    def __init__(self, cell_xyz, cell_dofs, rt_sign, nfree4, dir_vals, phys, faces=None, device=0):
+    def __init__(self, cell_xyz, cell_dofs, rt_sign, nfree4, dir_vals, phys, faces=None, device=0,
+                 cell_verts=None, block_style=0):
         self.lib = load()
         self._keep = []
         k = lambda a, dt: self._keep.append(np.ascontiguousarray(a, dtype=dt)) or self._keep[-1]
@@ -132,9 +134,12 @@ class Context:
         ft = k(faces.get("tau", np.zeros(0)), np.float64)
         fcw = k(faces.get("cw", np.zeros(0)), np.float64)
         fjw = k(faces.get("jw", np.zeros(0)), np.float64)
+        cv = k(cell_verts, np.int32) if cell_verts is not None else None
         m = MhdMesh(ncells=dofs.shape[0], nfaces_tw=fc.size, cell_xyz=_ptr(xyz), tw_face_cell=_ptr(fc),
-                    tw_face_lface=_ptr(fl), tw_tau=_ptr(ft), tw_cw=_ptr(fcw), tw_jw=_ptr(fjw))
+                    tw_face_lface=_ptr(fl), tw_tau=_ptr(ft), tw_cw=_ptr(fcw), tw_jw=_ptr(fjw),
+                    cell_verts=_ptr(cv), nverts=int(cv.max()) + 1 if cv is not None else 0)
         s = MhdSpaces()
+        s.block_style = int(block_style)
         s.nfree[:] = [int(v) for v in nfree4]
         s.ndir = int(len(dir_vals))
         s.cell_dofs, s.rt_sign, s.dir_vals = _ptr(dofs), _ptr(sign), _ptr(dv)
@@ -225,4 +230,5 @@ def context_from_spaces(sp, phys, dir_vals=None, device=0) -> Context:
                      jw=np.concatenate([np.full(t["cell"].size, t["jw"]) for t in tw]))
     nf = [sp.nfree["u"], sp.nfree["p"], sp.nfree["j"], sp.nfree["phi"]]
     dv = sp.dir_vals if dir_vals is None else dir_vals
-    return Context(sp.mesh.cell_xyz(), sp.cell_dofs, sp.rt_sign, nf, dv, phys, faces, device)
+    return Context(sp.mesh.cell_xyz(), sp.cell_dofs, sp.rt_sign, nf, dv, phys, faces, device,
+                   cell_verts=sp.cell_verts, block_style=1 if sp.style == "badia2024" else 0)

@@ -106,6 +106,18 @@ int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, co
     c->err = "spaces: dir_vals missing";
     return -1;
   }
+  c->block_style = spaces->block_style;
+  c->h_dofs.assign(spaces->cell_dofs, spaces->cell_dofs + ntab);
+  c->h_xyz.assign(mesh->cell_xyz, mesh->cell_xyz + (size_t)c->ncells * 24);
+  if (mesh->cell_verts) {
+    c->nverts = mesh->nverts;
+    c->h_verts.assign(mesh->cell_verts, mesh->cell_verts + (size_t)c->ncells * 8);
+    for (int32_t v : c->h_verts)
+      if (v < 0 || v >= c->nverts) {
+        c->err = "mesh: cell_verts entry out of range";
+        return -1;
+      }
+  }
   if ((rc = dev_upload(c, &c->d_xyz, mesh->cell_xyz, (size_t)c->ncells * 24))) return rc;
   if ((rc = dev_upload(c, &c->d_dofs, spaces->cell_dofs, (size_t)ntab))) return rc;
   if (spaces->rt_sign) {

@@ -90,9 +90,14 @@ struct mhd_ctx {
   mhd::FaceTables *d_fref = nullptr;
   // scratch
   double *d_x = nullptr, *d_r = nullptr;
-  // host copies needed by the API
+  // host copies needed by the API and the solver set-up
   std::vector<int64_t> h_rowptr;
   std::vector<int32_t> h_colind;
+  std::vector<int32_t> h_dofs;     // [ncells][129]
+  std::vector<int32_t> h_verts;    // [ncells][8] (empty: no vertex-star smoother)
+  std::vector<double> h_xyz;       // [ncells][24]
+  int32_t nverts = 0;
+  int32_t block_style = 0;
   // solver workspace (allocated lazily)
   struct Solver *solver = nullptr;
 };

@@ -1,28 +1,871 @@
-// Newton + FGMRES + block-triangular preconditioner (placeholder until the
-// Krylov layer lands).
+// Newton + FGMRES + upper block-triangular preconditioner (Badia2024Solver,
+// src/Solvers/badia2024.jl:2-48) on the GPU.
+//
+// Unknowns are numbered in the Badia2024 block style [u; j], p, phi
+// (src/fespaces.jl:8), so the [u,j] block A11 is the leading n1 x n1 block and,
+// with rows' columns sorted, every row of A11 is a prefix of its CSR row.
+//
+// Preconditioner, applied to v = (v1, vp, vphi):
+//   z_phi = S_phi^-1 v_phi,  S_phi = alpha_phi M_phi,  alpha_phi = -1/(1 + zeta_j)
+//   z_p   = S_p^-1 v_p,      S_p   = alpha_p M_p,      alpha_p   = -1/(beta + zeta_u)
+//   z1    ~ A11^-1 (v1 - A12 z_p - A13 z_phi)
+// M_p (dP1) and M_phi (dQ1) are cell-block diagonal, so their inverses are
+// exact cell-local 4x4 / 8x8 blocks (the reference reaches the same fixed point
+// with CG-Jacobi).  A11^-1 is approximated by `inner_iters` steps of GMRES on
+// A11, right-preconditioned by additive Schwarz over vertex-star patches (the
+// patch of a vertex holds every u / j dof whose support lies in the cells
+// around it); the dense patch inverses come from batched LU.
+//
+// Precision: precision 0 runs everything in FP64.  Precision 1 keeps FP64 for
+// the residual assembly, the outer FGMRES (the refinement residual, its
+// classical Gram-Schmidt with reorthogonalisation, the Newton update) and the
+// p / phi Schur solves, and runs the inner A11 GMRES in FP32 with an FP32 copy
+// of A11 and BF16 patch inverses.
+#include <hip/hip_bf16.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstring>
+
 #include "mhd_internal.h"
 
-void mhd_solver_free(mhd_ctx *c) { (void)c; }
+using namespace mhd;
+
+namespace {
+
+constexpr int TB = 256;
+
+inline int nblocks(int64_t n, int per = TB) { return (int)std::min<int64_t>((n + per - 1) / per, 1 << 20); }
+
+template <class T>
+__device__ inline double todbl(T v) { return (double)v; }
+template <>
+__device__ inline double todbl<__hip_bfloat16>(__hip_bfloat16 v) { return (double)__bfloat162float(v); }
+
+// ---------------------------------------------------------------- vectors
+template <class T>
+__global__ void k_fill(int64_t n, T v, T *__restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = v;
+}
+template <class T>
+__global__ void k_axpy(int64_t n, double a, const T *__restrict__ x, T *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (T)((double)y[i] + a * (double)x[i]);
+}
+template <class T>
+__global__ void k_scale_copy(int64_t n, double a, const T *__restrict__ x, T *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (T)(a * (double)x[i]);
+}
+template <class S, class D>
+__global__ void k_convert(int64_t n, const S *__restrict__ x, D *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (D)x[i];
+}
+__global__ void k_to_bf16(int64_t n, const double *__restrict__ x, __hip_bfloat16 *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = __float2bfloat16((float)x[i]);
+}
+
+__device__ inline double block_sum(double v) {
+  __shared__ double red[TB / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < TB / 64; ++k) s += red[k];
+  __syncthreads();
+  return s;
+}
+
+// part[k * gridDim.x + b] = sum over block b's slice of V_k . w   (grid: x = slices, y = k)
+template <class T>
+__global__ __launch_bounds__(TB) void k_mdot(const T *__restrict__ V, int64_t ldv, const T *__restrict__ w, int64_t n,
+                                             double *__restrict__ part) {
+  const int k = blockIdx.y;
+  const T *v = V + (int64_t)k * ldv;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)TB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TB)
+    acc += (double)v[i] * (double)w[i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) part[(int64_t)k * gridDim.x + blockIdx.x] = acc;
+}
+__global__ __launch_bounds__(TB) void k_mdot_final(int nb, const double *__restrict__ part, double *__restrict__ out) {
+  const int k = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += TB) acc += part[(int64_t)k * nb + b];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) out[k] = acc;
+}
+// w -= sum_k h[k] V_k  (h on device)
+template <class T>
+__global__ __launch_bounds__(TB) void k_gs_update(int K, const T *__restrict__ V, int64_t ldv,
+                                                  const double *__restrict__ h, T *__restrict__ w, int64_t n) {
+  __shared__ double hs[128];
+  for (int k = threadIdx.x; k < K; k += TB) hs[k] = h[k];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)TB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TB) {
+    double acc = (double)w[i];
+    for (int k = 0; k < K; ++k) acc -= hs[k] * (double)V[(int64_t)k * ldv + i];
+    w[i] = (T)acc;
+  }
+}
+// x = beta * x + sum_k y[k] Z_k  (y on device)
+template <class T>
+__global__ __launch_bounds__(TB) void k_lincomb(int K, const T *__restrict__ Z, int64_t ldz, const double *__restrict__ y,
+                                                double beta, T *__restrict__ x, int64_t n) {
+  __shared__ double ys[128];
+  for (int k = threadIdx.x; k < K; k += TB) ys[k] = y[k];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)TB + threadIdx.x; i < n; i += (int64_t)gridDim.x * TB) {
+    double acc = beta == 0.0 ? 0.0 : beta * (double)x[i];
+    for (int k = 0; k < K; ++k) acc += ys[k] * (double)Z[(int64_t)k * ldz + i];
+    x[i] = (T)acc;
+  }
+}
+
+// ---------------------------------------------------------------- SpMV
+// MODE 0: full rows.  MODE 1: A11 prefix [rowptr, split).  MODE 2: y = v - suffix [split, rowptr+1) . x
+template <class MV, class TX, class TY, int LPR, int MODE>
+__global__ __launch_bounds__(256) void k_spmv_seg(int64_t nrows, const int64_t *__restrict__ rowptr,
+                                                  const int64_t *__restrict__ split, const int32_t *__restrict__ colind,
+                                                  const MV *__restrict__ val, const TX *__restrict__ x,
+                                                  const TY *__restrict__ v, TY *__restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gid / LPR;
+  const int lane = threadIdx.x % LPR;
+  double acc = 0.0;
+  if (row < nrows) {
+    int64_t b = rowptr[row], e = rowptr[row + 1];
+    if (MODE == 1) e = split[row];
+    if (MODE == 2) b = split[row];
+    for (int64_t p = b + lane; p < e; p += LPR) acc += (double)val[p] * (double)x[colind[p]];
+  }
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
+  if (row < nrows && lane == 0) {
+    if (MODE == 2) y[row] = (TY)((double)v[row] - acc);
+    else y[row] = (TY)acc;
+  }
+}
+
+__global__ void k_split(int64_t n1, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                        int64_t *__restrict__ split) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n1; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = rowptr[i], hi = rowptr[i + 1];
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (colind[mid] < n1) lo = mid + 1;
+      else hi = mid;
+    }
+    split[i] = lo;
+  }
+}
+
+// ---------------------------------------------------------------- Schur blocks
+// x_p = Sp_c r_p and x_phi = Sphi_c r_phi for every cell (thread per (cell, row)).
+template <class T>
+__global__ void k_schur(int ncells, const int32_t *__restrict__ pfirst, const int32_t *__restrict__ phifirst,
+                        const double *__restrict__ Sp, const double *__restrict__ Sphi, const T *__restrict__ r,
+                        T *__restrict__ x) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ncells * 12) return;
+  const int c = (int)(t / 12), k = (int)(t % 12);
+  if (k < 4) {
+    const int32_t p0 = pfirst[c];
+    const double *S = Sp + (int64_t)c * 16 + 4 * k;
+    double acc = 0.0;
+    for (int l = 0; l < 4; ++l) acc += S[l] * (double)r[p0 + l];
+    x[p0 + k] = (T)acc;
+  } else {
+    const int kk = k - 4;
+    const int32_t f0 = phifirst[c];
+    const double *S = Sphi + (int64_t)c * 64 + 8 * kk;
+    double acc = 0.0;
+    for (int l = 0; l < 8; ++l) acc += S[l] * (double)r[f0 + l];
+    x[f0 + kk] = (T)acc;
+  }
+}
+
+// ---------------------------------------------------------------- patches
+// A_P (column-major, PS x PS) = A11[P, P]; padding rows/cols get the identity.
+__global__ void k_patch_extract(int npatch, int PS, const int32_t *__restrict__ psize, const int32_t *__restrict__ pidx,
+                                const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                                const double *__restrict__ val, double *__restrict__ A) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)npatch * PS) return;
+  const int pch = (int)(t / PS), a = (int)(t % PS);
+  double *Ap = A + (int64_t)pch * PS * PS;
+  const int sz = psize[pch];
+  if (a >= sz) {
+    Ap[a + (int64_t)a * PS] = 1.0;
+    return;
+  }
+  const int32_t *ids = pidx + (int64_t)pch * PS;
+  const int32_t g = ids[a];
+  int64_t p = rowptr[g];
+  const int64_t e = rowptr[g + 1];
+  int b = 0;
+  while (p < e && b < sz) {
+    const int32_t cp = colind[p], cb = ids[b];
+    if (cp < cb) ++p;
+    else if (cp > cb) ++b;
+    else {
+      Ap[a + (int64_t)b * PS] = val[p];
+      ++p;
+      ++b;
+    }
+  }
+}
+
+// y[P] += A_P^-1 r[P]  (one workgroup per patch, inverse column-major)
+template <class MP, class T>
+__global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const int32_t *__restrict__ psize,
+                                                     const int32_t *__restrict__ pidx, const MP *__restrict__ Ainv,
+                                                     const T *__restrict__ r, T *__restrict__ y) {
+  __shared__ double rl[256];
+  const int pch = blockIdx.x;
+  const int sz = psize[pch];
+  const int32_t *ids = pidx + (int64_t)pch * PS;
+  const int t = threadIdx.x;
+  if (t < sz) rl[t] = (double)r[ids[t]];
+  __syncthreads();
+  if (t < sz) {
+    const MP *Ap = Ainv + (int64_t)pch * PS * PS + t;
+    double acc = 0.0;
+    for (int b = 0; b < sz; ++b) acc += todbl(Ap[(int64_t)b * PS]) * rl[b];
+    atomicAdd(&y[ids[t]], (T)acc);
+  }
+}
+
+}  // namespace
+
+// ==================================================================== host side
+struct Solver {
+  int64_t n = 0, n1 = 0, np = 0, nphi = 0;
+  int64_t *d_split = nullptr;
+  int32_t *d_pfirst = nullptr, *d_phifirst = nullptr;
+  std::vector<double> h_Mp, h_Mphi;   // cell mass inverses (unscaled)
+  double *d_Sp = nullptr, *d_Sphi = nullptr;
+  double alpha_p = 0.0, alpha_phi = 0.0;
+  // patches
+  int npatch = 0, PS = 0;
+  int32_t *d_psize = nullptr, *d_pidx = nullptr;
+  double *d_pA = nullptr;
+  float *d_pA32 = nullptr;
+  __hip_bfloat16 *d_pA16 = nullptr;
+  rocblas_int *d_ipiv = nullptr, *d_info = nullptr;
+  rocblas_handle rb = nullptr;
+  // mixed precision copy of A11 values
+  float *d_val32 = nullptr;
+  // Krylov workspace
+  int m = 0, kin = 0;
+  double *V = nullptr, *Z = nullptr, *w = nullptr, *b = nullptr, *dx = nullptr, *t1 = nullptr;
+  void *V1 = nullptr, *Z1 = nullptr, *w1 = nullptr, *x1 = nullptr, *r1 = nullptr;  // inner, n1-long, FP64 or FP32
+  int inner_prec = -1;
+  double *d_part = nullptr, *d_dots = nullptr, *d_y = nullptr;
+  double *h_dots = nullptr;  // pinned
+  int nb_dot = 0;
+};
+
+namespace {
+
+template <class T>
+int dmalloc(mhd_ctx *c, T **p, size_t count) {
+  MHD_HIP(hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * std::max<size_t>(count, 1)));
+  return 0;
+}
+
+// Gauss-Jordan inverse of a small SPD matrix (host, no pivoting needed).
+void inv_spd(int n, const double *A, double *Ai) {
+  std::vector<double> M(n * 2 * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < 2 * n; ++j) M[i * 2 * n + j] = j < n ? A[i * n + j] : (j - n == i ? 1.0 : 0.0);
+  for (int c = 0; c < n; ++c) {
+    double d = 1.0 / M[c * 2 * n + c];
+    for (int j = 0; j < 2 * n; ++j) M[c * 2 * n + j] *= d;
+    for (int r = 0; r < n; ++r)
+      if (r != c) {
+        double f = M[r * 2 * n + c];
+        if (f != 0.0)
+          for (int j = 0; j < 2 * n; ++j) M[r * 2 * n + j] -= f * M[c * 2 * n + j];
+      }
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) Ai[i * n + j] = M[i * 2 * n + n + j];
+}
+
+// Cell mass matrices of dP1 and dQ1 (3x3x3 Gauss, the degree-4/5 measures of
+// Badia2024Solver, src/Solvers/badia2024.jl:8-15) and their inverses.
+void cell_mass_inverses(mhd_ctx *c, std::vector<double> &Mp, std::vector<double> &Mphi) {
+  mhd::RefTables *T = new mhd::RefTables;
+  mhd::build_ref_tables(T);
+  Mp.assign((size_t)c->ncells * 16, 0.0);
+  Mphi.assign((size_t)c->ncells * 64, 0.0);
+  for (int32_t cc = 0; cc < c->ncells; ++cc) {
+    const double *X = &c->h_xyz[(size_t)cc * 24];
+    double mp[16] = {0}, mf[64] = {0};
+    for (int q = 0; q < NQ; ++q) {
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int v = 0; v < 8; ++v)
+        for (int i = 0; i < 3; ++i)
+          for (int l = 0; l < 3; ++l) J[i][l] += X[3 * v + i] * T->dq1[q][v][l];
+      const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                         J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      const double W = T->w[q] * std::fabs(det);
+      for (int k = 0; k < 4; ++k)
+        for (int l = 0; l < 4; ++l) mp[4 * k + l] += W * T->p1[q][k] * T->p1[q][l];
+      for (int k = 0; k < 8; ++k)
+        for (int l = 0; l < 8; ++l) mf[8 * k + l] += W * T->q1[q][k] * T->q1[q][l];
+    }
+    inv_spd(4, mp, &Mp[(size_t)cc * 16]);
+    inv_spd(8, mf, &Mphi[(size_t)cc * 64]);
+  }
+  delete T;
+}
+
+// Vertex-star patches: dofs of u and j (A11 numbering) whose entity contains the vertex.
+void build_patches(mhd_ctx *c, std::vector<int32_t> &psize, std::vector<int32_t> &pidx, int &PS) {
+  const int64_t n1 = (int64_t)c->nfree[0] + c->nfree[2];
+  std::vector<std::vector<int32_t>> lists(c->nverts);
+  for (int32_t cc = 0; cc < c->ncells; ++cc) {
+    const int32_t *cd = &c->h_dofs[(size_t)cc * NDOF];
+    for (int lv = 0; lv < 8; ++lv) {
+      const int vi = lv & 1, vj = (lv >> 1) & 1, vk = lv >> 2;
+      auto &L = lists[c->h_verts[(size_t)cc * 8 + lv]];
+      for (int a = 0; a < 27; ++a) {
+        const int i = a % 3, j = (a / 3) % 3, k = a / 9;
+        if (std::abs(i - 2 * vi) > 1 || std::abs(j - 2 * vj) > 1 || std::abs(k - 2 * vk) > 1) continue;
+        for (int comp = 0; comp < 3; ++comp) {
+          const int32_t g = cd[comp * 27 + a];
+          if (g >= 0 && g < n1) L.push_back(g);
+        }
+      }
+      for (int r = 0; r < 36; ++r) {
+        bool in = true;
+        if (r < 24) {
+          const int f = r >> 2, d = 2 - f / 2, side = f & 1;
+          const int vc = d == 0 ? vi : (d == 1 ? vj : vk);
+          in = vc == side;
+        }
+        if (!in) continue;
+        const int32_t g = cd[OFF_J + r];
+        if (g >= 0 && g < n1) L.push_back(g);
+      }
+    }
+  }
+  PS = 0;
+  int np = 0;
+  for (auto &L : lists) {
+    std::sort(L.begin(), L.end());
+    L.erase(std::unique(L.begin(), L.end()), L.end());
+    if (!L.empty()) {
+      PS = std::max<int>(PS, (int)L.size());
+      ++np;
+    }
+  }
+  psize.assign(np, 0);
+  pidx.assign((size_t)np * PS, -1);
+  int k = 0;
+  for (auto &L : lists) {
+    if (L.empty()) continue;
+    psize[k] = (int)L.size();
+    std::copy(L.begin(), L.end(), pidx.begin() + (size_t)k * PS);
+    ++k;
+  }
+}
+
+int solver_init(mhd_ctx *c, const mhd_solver_opts *o) {
+  if (c->solver) return 0;
+  if (c->block_style != 1) {
+    c->err = "mhd_newton_solve needs the Badia2024 block numbering (spaces.block_style = 1)";
+    return -1;
+  }
+  if (c->h_verts.empty()) {
+    c->err = "mhd_newton_solve needs mesh.cell_verts for the vertex-star smoother";
+    return -1;
+  }
+  Solver *S = new Solver();
+  c->solver = S;
+  S->n = c->n;
+  S->n1 = (int64_t)c->nfree[0] + c->nfree[2];
+  S->np = c->nfree[1];
+  S->nphi = c->nfree[3];
+  // p / phi dofs: consecutive per cell
+  std::vector<int32_t> pf(c->ncells), ff(c->ncells);
+  for (int32_t cc = 0; cc < c->ncells; ++cc) {
+    const int32_t *cd = &c->h_dofs[(size_t)cc * NDOF];
+    pf[cc] = cd[OFF_P];
+    ff[cc] = cd[OFF_PHI];
+    for (int k = 0; k < 4; ++k)
+      if (cd[OFF_P + k] != pf[cc] + k || pf[cc] < S->n1 || pf[cc] + 4 > S->n1 + S->np) {
+        c->err = "p dofs of a cell must be consecutive and inside the p block";
+        return -1;
+      }
+    for (int k = 0; k < 8; ++k)
+      if (cd[OFF_PHI + k] != ff[cc] + k || ff[cc] < S->n1 + S->np || ff[cc] + 8 > S->n) {
+        c->err = "phi dofs of a cell must be consecutive and inside the phi block";
+        return -1;
+      }
+  }
+  int rc;
+  if ((rc = dmalloc(c, &S->d_pfirst, c->ncells))) return rc;
+  if ((rc = dmalloc(c, &S->d_phifirst, c->ncells))) return rc;
+  MHD_HIP(hipMemcpy(S->d_pfirst, pf.data(), sizeof(int32_t) * c->ncells, hipMemcpyHostToDevice));
+  MHD_HIP(hipMemcpy(S->d_phifirst, ff.data(), sizeof(int32_t) * c->ncells, hipMemcpyHostToDevice));
+  cell_mass_inverses(c, S->h_Mp, S->h_Mphi);
+  if ((rc = dmalloc(c, &S->d_Sp, S->h_Mp.size()))) return rc;
+  if ((rc = dmalloc(c, &S->d_Sphi, S->h_Mphi.size()))) return rc;
+  if ((rc = dmalloc(c, &S->d_split, S->n1))) return rc;
+  hipLaunchKernelGGL(k_split, dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, c->d_rowptr, c->d_colind,
+                     S->d_split);
+  MHD_HIP(hipGetLastError());
+  // patches
+  std::vector<int32_t> psize, pidx;
+  build_patches(c, psize, pidx, S->PS);
+  S->npatch = (int)psize.size();
+  if (S->PS > 256) {
+    c->err = "vertex patch larger than 256 dofs";
+    return -1;
+  }
+  if ((rc = dmalloc(c, &S->d_psize, psize.size()))) return rc;
+  if ((rc = dmalloc(c, &S->d_pidx, pidx.size()))) return rc;
+  MHD_HIP(hipMemcpy(S->d_psize, psize.data(), sizeof(int32_t) * psize.size(), hipMemcpyHostToDevice));
+  MHD_HIP(hipMemcpy(S->d_pidx, pidx.data(), sizeof(int32_t) * pidx.size(), hipMemcpyHostToDevice));
+  const size_t pelems = (size_t)S->npatch * S->PS * S->PS;
+  if ((rc = dmalloc(c, &S->d_pA, pelems))) return rc;
+  if ((rc = dmalloc(c, &S->d_ipiv, (size_t)S->npatch * S->PS))) return rc;
+  if ((rc = dmalloc(c, &S->d_info, (size_t)S->npatch))) return rc;
+  if (rocblas_create_handle(&S->rb) != rocblas_status_success) {
+    c->err = "rocblas_create_handle failed";
+    return -2;
+  }
+  rocblas_set_stream(S->rb, c->stream);
+  // Krylov workspace
+  S->m = std::max(2, std::min(o->restart, 120));
+  S->kin = std::max(0, std::min(o->inner_iters, 120));
+  const size_t n = (size_t)S->n;
+  if ((rc = dmalloc(c, &S->V, (S->m + 1) * n))) return rc;
+  if ((rc = dmalloc(c, &S->Z, S->m * n))) return rc;
+  if ((rc = dmalloc(c, &S->w, n))) return rc;
+  if ((rc = dmalloc(c, &S->b, n))) return rc;
+  if ((rc = dmalloc(c, &S->dx, n))) return rc;
+  if ((rc = dmalloc(c, &S->t1, n))) return rc;
+  S->nb_dot = 512;
+  if ((rc = dmalloc(c, &S->d_part, (size_t)S->nb_dot * 256))) return rc;
+  if ((rc = dmalloc(c, &S->d_dots, 512))) return rc;
+  if ((rc = dmalloc(c, &S->d_y, 256))) return rc;
+  MHD_HIP(hipHostMalloc(reinterpret_cast<void **>(&S->h_dots), sizeof(double) * 512, hipHostMallocDefault));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int inner_alloc(mhd_ctx *c, int prec) {
+  Solver *S = c->solver;
+  if (S->inner_prec == prec) return 0;
+  void *ptrs[] = {S->V1, S->Z1, S->w1, S->x1, S->r1};
+  for (void *p : ptrs)
+    if (p) MHD_HIP(hipFree(p));
+  const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
+  const size_t n1 = (size_t)S->n1;
+  MHD_HIP(hipMalloc(&S->V1, es * (S->kin + 1) * n1 + es));
+  MHD_HIP(hipMalloc(&S->Z1, es * std::max(S->kin, 1) * n1 + es));
+  MHD_HIP(hipMalloc(&S->w1, es * n1 + es));
+  MHD_HIP(hipMalloc(&S->x1, es * n1 + es));
+  MHD_HIP(hipMalloc(&S->r1, es * n1 + es));
+  if (prec == 1) {
+    if (!S->d_val32) MHD_HIP(hipMalloc(&S->d_val32, sizeof(float) * std::max<int64_t>(c->nnz, 1)));
+    if (!S->d_pA16) MHD_HIP(hipMalloc(&S->d_pA16, sizeof(__hip_bfloat16) * (size_t)S->npatch * S->PS * S->PS));
+  }
+  S->inner_prec = prec;
+  return 0;
+}
+
+// ---- small host helpers around the device reductions ----------------------
+template <class T>
+int mdot(mhd_ctx *c, const T *V, int64_t ldv, int K, const T *w, int64_t n, double *d_out) {
+  Solver *S = c->solver;
+  hipLaunchKernelGGL((k_mdot<T>), dim3(S->nb_dot, K), dim3(TB), 0, c->stream, V, ldv, w, n, S->d_part);
+  hipLaunchKernelGGL(k_mdot_final, dim3(K), dim3(TB), 0, c->stream, S->nb_dot, S->d_part, d_out);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+// CGS2 Arnoldi step: orthogonalise w against V[0..K), return h (K) and ||w|| (host), w normalised into V[K].
+template <class T>
+int arnoldi(mhd_ctx *c, T *V, int64_t ldv, int K, T *w, int64_t n, double *h, double *nrm) {
+  Solver *S = c->solver;
+  int rc;
+  double *d1 = S->d_dots, *d2 = S->d_dots + 128, *dn = S->d_dots + 256;
+  if ((rc = mdot(c, V, ldv, K, w, n, d1))) return rc;
+  hipLaunchKernelGGL((k_gs_update<T>), dim3(nblocks(n)), dim3(TB), 0, c->stream, K, V, ldv, d1, w, n);
+  if ((rc = mdot(c, V, ldv, K, w, n, d2))) return rc;
+  hipLaunchKernelGGL((k_gs_update<T>), dim3(nblocks(n)), dim3(TB), 0, c->stream, K, V, ldv, d2, w, n);
+  if ((rc = mdot(c, w, 0, 1, w, n, dn))) return rc;
+  MHD_HIP(hipMemcpyAsync(S->h_dots, S->d_dots, sizeof(double) * 257, hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < K; ++k) h[k] = S->h_dots[k] + S->h_dots[128 + k];
+  *nrm = std::sqrt(std::max(S->h_dots[256], 0.0));
+  T *vk = V + (int64_t)K * ldv;
+  if (*nrm > 0.0)
+    hipLaunchKernelGGL((k_scale_copy<T>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, 1.0 / *nrm, w, vk);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class T>
+int vnorm(mhd_ctx *c, const T *x, int64_t n, double *out) {
+  Solver *S = c->solver;
+  int rc = mdot(c, x, 0, 1, x, n, S->d_dots + 300);
+  if (rc) return rc;
+  MHD_HIP(hipMemcpyAsync(S->h_dots + 300, S->d_dots + 300, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  *out = std::sqrt(std::max(S->h_dots[300], 0.0));
+  return 0;
+}
+
+// Least-squares solve of the (k+1) x k Hessenberg system via Givens rotations.
+void hess_solve(int k, const std::vector<double> &H, int ldh, double beta, std::vector<double> &y) {
+  std::vector<double> R(H), g(k + 1, 0.0);
+  g[0] = beta;
+  for (int j = 0; j < k; ++j) {
+    for (int i = 0; i < j; ++i) {
+      // already rotated in place below
+    }
+  }
+  // apply rotations column by column
+  std::vector<double> cs(k), sn(k);
+  for (int j = 0; j < k; ++j) {
+    for (int i = 0; i < j; ++i) {
+      const double a = R[i * ldh + j], b = R[(i + 1) * ldh + j];
+      R[i * ldh + j] = cs[i] * a + sn[i] * b;
+      R[(i + 1) * ldh + j] = -sn[i] * a + cs[i] * b;
+    }
+    const double a = R[j * ldh + j], b = R[(j + 1) * ldh + j];
+    const double d = std::hypot(a, b);
+    cs[j] = d > 0 ? a / d : 1.0;
+    sn[j] = d > 0 ? b / d : 0.0;
+    R[j * ldh + j] = d;
+    R[(j + 1) * ldh + j] = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+  }
+  y.assign(k, 0.0);
+  for (int i = k - 1; i >= 0; --i) {
+    double s = g[i];
+    for (int j = i + 1; j < k; ++j) s -= R[i * ldh + j] * y[j];
+    y[i] = R[i * ldh + i] != 0.0 ? s / R[i * ldh + i] : 0.0;
+  }
+}
+
+// Additive Schwarz over the vertex patches: y = sum_P R_P^T A_P^-1 R_P r
+template <class T>
+int patch_as(mhd_ctx *c, const T *r, T *y, int prec) {
+  Solver *S = c->solver;
+  hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, (T)0, y);
+  if (prec == 0)
+    hipLaunchKernelGGL((k_patch_apply<double, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch, S->PS,
+                       S->d_psize, S->d_pidx, S->d_pA, r, y);
+  else
+    hipLaunchKernelGGL((k_patch_apply<__hip_bfloat16, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch,
+                       S->PS, S->d_psize, S->d_pidx, S->d_pA16, r, y);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class T>
+int spmv_a11(mhd_ctx *c, const T *x, T *y, int prec) {
+  Solver *S = c->solver;
+  constexpr int LPR = 16;
+  const int grid = (int)((S->n1 * LPR + 255) / 256);
+  if (prec == 0)
+    hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 1>), dim3(grid), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
+                       S->d_split, c->d_colind, c->d_val, x, (const T *)nullptr, y);
+  else
+    hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 1>), dim3(grid), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
+                       S->d_split, c->d_colind, S->d_val32, x, (const T *)nullptr, y);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+// x1 ~ A11^-1 r1 by kin steps of right-preconditioned GMRES from x = 0.
+template <class T>
+int inner_solve(mhd_ctx *c, int prec) {
+  Solver *S = c->solver;
+  const int64_t n1 = S->n1;
+  T *V1 = (T *)S->V1, *Z1 = (T *)S->Z1, *w1 = (T *)S->w1, *x1 = (T *)S->x1, *r1 = (T *)S->r1;
+  int rc;
+  if (S->kin == 0) return patch_as<T>(c, r1, x1, prec);
+  double beta;
+  if ((rc = vnorm(c, r1, n1, &beta))) return rc;
+  hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(n1)), dim3(TB), 0, c->stream, n1, (T)0, x1);
+  if (beta == 0.0) return 0;
+  hipLaunchKernelGGL((k_scale_copy<T>), dim3(nblocks(n1)), dim3(TB), 0, c->stream, n1, 1.0 / beta, r1, V1);
+  const int ldh = S->kin;
+  std::vector<double> H((S->kin + 1) * ldh, 0.0), h(S->kin + 1), y;
+  int k = 0;
+  for (int j = 0; j < S->kin; ++j) {
+    if ((rc = patch_as<T>(c, V1 + j * n1, Z1 + j * n1, prec))) return rc;
+    if ((rc = spmv_a11<T>(c, Z1 + j * n1, w1, prec))) return rc;
+    double nrm;
+    if ((rc = arnoldi<T>(c, V1, n1, j + 1, w1, n1, h.data(), &nrm))) return rc;
+    for (int i = 0; i <= j; ++i) H[i * ldh + j] = h[i];
+    H[(j + 1) * ldh + j] = nrm;
+    k = j + 1;
+    if (nrm <= 1e-300) break;
+  }
+  hess_solve(k, H, ldh, beta, y);
+  MHD_HIP(hipMemcpyAsync(S->d_y, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL((k_lincomb<T>), dim3(nblocks(n1)), dim3(TB), 0, c->stream, k, Z1, n1, S->d_y, 0.0, x1, n1);
+  MHD_HIP(hipGetLastError());
+  MHD_HIP(hipStreamSynchronize(c->stream));  // y is a pageable host buffer
+  return 0;
+}
+
+// z = P^-1 v (FP64 in/out)
+int precond(mhd_ctx *c, const double *v, double *z, int prec) {
+  Solver *S = c->solver;
+  int rc;
+  hipLaunchKernelGGL((k_schur<double>), dim3(nblocks((int64_t)c->ncells * 12)), dim3(TB), 0, c->stream, c->ncells,
+                     S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, v, z);
+  // rhs1 = v1 - A12 z_p - A13 z_phi (FP64)
+  constexpr int LPR = 16;
+  const int grid = (int)((S->n1 * LPR + 255) / 256);
+  if (prec == 0) {
+    hipLaunchKernelGGL((k_spmv_seg<double, double, double, LPR, 2>), dim3(grid), dim3(256), 0, c->stream, S->n1,
+                       c->d_rowptr, S->d_split, c->d_colind, c->d_val, z, v, (double *)S->r1);
+    MHD_HIP(hipGetLastError());
+    if ((rc = inner_solve<double>(c, 0))) return rc;
+    MHD_HIP(hipMemcpyAsync(z, S->x1, sizeof(double) * S->n1, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    hipLaunchKernelGGL((k_spmv_seg<double, double, double, LPR, 2>), dim3(grid), dim3(256), 0, c->stream, S->n1,
+                       c->d_rowptr, S->d_split, c->d_colind, c->d_val, z, v, S->t1);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, S->t1,
+                       (float *)S->r1);
+    MHD_HIP(hipGetLastError());
+    if ((rc = inner_solve<float>(c, 1))) return rc;
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1,
+                       (const float *)S->x1, z);
+  }
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+// Per-Jacobian set-up: Schur scalings, FP32 copy, patch factorisation.
+int precond_setup(mhd_ctx *c, int prec) {
+  Solver *S = c->solver;
+  int rc;
+  if ((rc = inner_alloc(c, prec))) return rc;
+  const double ap = -1.0 / (c->phys.beta + c->phys.zeta_u);
+  const double aphi = -1.0 / (1.0 + c->phys.zeta_j);
+  if (ap != S->alpha_p || aphi != S->alpha_phi) {
+    std::vector<double> sp(S->h_Mp), sf(S->h_Mphi);
+    for (auto &v : sp) v /= ap;
+    for (auto &v : sf) v /= aphi;
+    MHD_HIP(hipMemcpy(S->d_Sp, sp.data(), sizeof(double) * sp.size(), hipMemcpyHostToDevice));
+    MHD_HIP(hipMemcpy(S->d_Sphi, sf.data(), sizeof(double) * sf.size(), hipMemcpyHostToDevice));
+    S->alpha_p = ap;
+    S->alpha_phi = aphi;
+  }
+  if (prec == 1)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(c->nnz)), dim3(TB), 0, c->stream, c->nnz, c->d_val,
+                       S->d_val32);
+  const size_t pelems = (size_t)S->npatch * S->PS * S->PS;
+  MHD_HIP(hipMemsetAsync(S->d_pA, 0, sizeof(double) * pelems, c->stream));
+  const int64_t nt = (int64_t)S->npatch * S->PS;
+  hipLaunchKernelGGL(k_patch_extract, dim3(nblocks(nt)), dim3(TB), 0, c->stream, S->npatch, S->PS, S->d_psize,
+                     S->d_pidx, c->d_rowptr, c->d_colind, c->d_val, S->d_pA);
+  MHD_HIP(hipGetLastError());
+  const rocblas_stride st = (rocblas_stride)S->PS * S->PS;
+  if (rocsolver_dgetrf_strided_batched(S->rb, S->PS, S->PS, S->d_pA, S->PS, st, S->d_ipiv, S->PS, S->d_info,
+                                       S->npatch) != rocblas_status_success ||
+      rocsolver_dgetri_strided_batched(S->rb, S->PS, S->d_pA, S->PS, st, S->d_ipiv, S->PS, S->d_info, S->npatch) !=
+          rocblas_status_success) {
+    c->err = "rocsolver batched LU of the vertex patches failed";
+    return -2;
+  }
+  if (prec == 1)
+    hipLaunchKernelGGL(k_to_bf16, dim3(nblocks((int64_t)pelems)), dim3(TB), 0, c->stream, (int64_t)pelems, S->d_pA,
+                       S->d_pA16);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+// Right-preconditioned restarted FGMRES (FP64) for J dx = b, dx = 0 on entry.
+int fgmres(mhd_ctx *c, const double *b, double *x, double tol, int maxit, int prec, int *iters, int verbose) {
+  Solver *S = c->solver;
+  const int64_t n = S->n;
+  const int m = S->m;
+  int rc, it = 0;
+  std::vector<double> H((m + 1) * m), h(m + 1), y, cs(m), sn(m), g(m + 1);
+  hipLaunchKernelGGL((k_fill<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, 0.0, x);
+  // r = b (x = 0)
+  MHD_HIP(hipMemcpyAsync(S->w, b, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  double beta;
+  if ((rc = vnorm(c, S->w, n, &beta))) return rc;
+  const double b0 = beta;
+  while (it < maxit && beta > tol) {
+    hipLaunchKernelGGL((k_scale_copy<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, 1.0 / beta, S->w, S->V);
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    double est = beta;
+    for (int j = 0; j < m && it < maxit; ++j) {
+      if ((rc = precond(c, S->V + j * n, S->Z + j * n, prec))) return rc;
+      if ((rc = mhd_launch_spmv(c, c->d_val, S->Z + j * n, S->w, c->stream))) return rc;
+      double nrm;
+      if ((rc = arnoldi<double>(c, S->V, n, j + 1, S->w, n, h.data(), &nrm))) return rc;
+      for (int i = 0; i <= j; ++i) H[i * m + j] = h[i];
+      H[(j + 1) * m + j] = nrm;
+      // incremental Givens for the residual estimate
+      for (int i = 0; i < j; ++i) {
+        const double a = H[i * m + j], bb = H[(i + 1) * m + j];
+        H[i * m + j] = cs[i] * a + sn[i] * bb;
+        H[(i + 1) * m + j] = -sn[i] * a + cs[i] * bb;
+      }
+      const double a = H[j * m + j], bb = H[(j + 1) * m + j];
+      const double d = std::hypot(a, bb);
+      cs[j] = d > 0 ? a / d : 1.0;
+      sn[j] = d > 0 ? bb / d : 0.0;
+      H[j * m + j] = d;
+      H[(j + 1) * m + j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      est = std::fabs(g[j + 1]);
+      ++it;
+      k = j + 1;
+      if (verbose > 1) fprintf(stderr, "    fgmres it %d  est %.3e\n", it, est / b0);
+      if (est <= tol || nrm <= 1e-300) break;
+    }
+    // back substitution on the rotated (upper triangular) H
+    y.assign(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int j = i + 1; j < k; ++j) s -= H[i * m + j] * y[j];
+      y[i] = s / H[i * m + i];
+    }
+    MHD_HIP(hipMemcpyAsync(S->d_y, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL((k_lincomb<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, k, S->Z, n, S->d_y, 1.0, x, n);
+    MHD_HIP(hipGetLastError());
+    // true residual for the restart: w = b - J x
+    if ((rc = mhd_launch_spmv(c, c->d_val, x, S->w, c->stream))) return rc;
+    hipLaunchKernelGGL((k_scale_copy<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, -1.0, S->w, S->w);
+    hipLaunchKernelGGL((k_axpy<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, 1.0, b, S->w);
+    MHD_HIP(hipGetLastError());
+    if ((rc = vnorm(c, S->w, n, &beta))) return rc;
+    if (verbose) fprintf(stderr, "  fgmres restart: it %d  true rel res %.3e\n", it, beta / b0);
+    if (k == 0) break;
+  }
+  *iters = it;
+  return beta <= tol ? 0 : 1;
+}
+
+double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+}  // namespace
+
+void mhd_solver_free(mhd_ctx *c) {
+  if (!c || !c->solver) return;
+  Solver *S = c->solver;
+  void *ptrs[] = {S->d_split, S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, S->d_psize, S->d_pidx, S->d_pA,
+                  S->d_pA32, S->d_pA16, S->d_ipiv, S->d_info, S->d_val32, S->V, S->Z, S->w, S->b, S->dx, S->t1,
+                  S->V1, S->Z1, S->w1, S->x1, S->r1, S->d_part, S->d_dots, S->d_y};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (S->h_dots) (void)hipHostFree(S->h_dots);
+  if (S->rb) rocblas_destroy_handle(S->rb);
+  delete S;
+  c->solver = nullptr;
+}
 
 extern "C" {
 
 void mhd_default_solver_opts(mhd_solver_opts *o) {
   std::memset(o, 0, sizeof(*o));
   o->max_newton = 20;
-  o->restart = 15;
-  o->max_krylov = 15;
+  o->restart = 40;
+  o->max_krylov = 400;
   o->precision = 0;
   o->rtol = 1e-6;
   o->atol = 1e-8;
   o->ls_rtol = 1e-7;
-  o->inner_iters = 4;
+  o->inner_iters = 8;
 }
 
-int mhd_newton_solve(mhd_ctx *c, double *x, const mhd_solver_opts *o, mhd_solve_info *info) {
-  (void)x; (void)o; (void)info;
-  if (c) c->err = "mhd_newton_solve: not available in this build";
-  return -5;
+int mhd_newton_solve(mhd_ctx *c, double *x_host, const mhd_solver_opts *o, mhd_solve_info *info) {
+  if (!c || !x_host || !o || !info) return -1;
+  std::memset(info, 0, sizeof(*info));
+  if (o->precision != 0 && o->precision != 1) {
+    c->err = "precision must be 0 (FP64) or 1 (mixed)";
+    return -1;
+  }
+  MHD_HIP(hipSetDevice(c->device));
+  int rc;
+  double t0 = now();
+  if ((rc = solver_init(c, o))) return rc;
+  Solver *S = c->solver;
+  if (S->m != std::max(2, std::min(o->restart, 120)) || S->kin != std::max(0, std::min(o->inner_iters, 120))) {
+    mhd_solver_free(c);
+    if ((rc = solver_init(c, o))) return rc;
+    S = c->solver;
+  }
+  info->t_setup += now() - t0;
+  const int64_t n = c->n;
+  MHD_HIP(hipMemcpyAsync(c->d_x, x_host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  double t = now();
+  if ((rc = mhd_launch_residual(c, c->d_x, c->d_r, c->stream))) return rc;
+  double r0;
+  if ((rc = vnorm(c, c->d_r, n, &r0))) return rc;
+  info->t_residual += now() - t;
+  info->residual0 = r0;
+  info->newton_residuals[0] = r0;
+  double rn = r0;
+  int it = 0;
+  int lin_fail = 0;
+  while (it < o->max_newton) {
+    if (rn <= std::max(o->atol, o->rtol * r0)) break;
+    t = now();
+    if ((rc = mhd_launch_jacobian(c, c->d_x, c->d_val, c->stream))) return rc;
+    MHD_HIP(hipStreamSynchronize(c->stream));
+    info->t_jacobian += now() - t;
+    t = now();
+    if ((rc = precond_setup(c, o->precision))) return rc;
+    MHD_HIP(hipStreamSynchronize(c->stream));
+    info->t_setup += now() - t;
+    t = now();
+    // b = -r
+    hipLaunchKernelGGL((k_scale_copy<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, -1.0, c->d_r, S->b);
+    int kits = 0;
+    rc = fgmres(c, S->b, S->dx, o->ls_rtol * rn, o->max_krylov, o->precision, &kits, o->verbose);
+    if (rc < 0) return rc;
+    if (rc > 0) lin_fail = 1;
+    hipLaunchKernelGGL((k_axpy<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, 1.0, S->dx, c->d_x);
+    MHD_HIP(hipStreamSynchronize(c->stream));
+    info->t_solve += now() - t;
+    if (it < 64) info->krylov_per_step[it] = kits;
+    info->krylov_iters += kits;
+    ++it;
+    t = now();
+    if ((rc = mhd_launch_residual(c, c->d_x, c->d_r, c->stream))) return rc;
+    if ((rc = vnorm(c, c->d_r, n, &rn))) return rc;
+    info->t_residual += now() - t;
+    if (it < 64) info->newton_residuals[it] = rn;
+    if (o->verbose) fprintf(stderr, "newton %d  |R| = %.6e  krylov %d\n", it, rn, kits);
+  }
+  MHD_HIP(hipMemcpyAsync(x_host, c->d_x, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  info->newton_iters = it;
+  info->residual = rn;
+  info->converged = rn <= std::max(o->atol, o->rtol * r0);
+  info->flags = lin_fail;
+  return info->converged ? 0 : 1;
 }
 
-}
+}  // extern "C"

@@ -69,6 +69,8 @@ class Spaces:
     face_ids: np.ndarray       # (ncells, 6) global face id
     face_dirichlet: np.ndarray  # (n_faces,) bool
     face_dof: np.ndarray       # (n_faces,) first field-local j id of the face's 4 dofs
+    cell_verts: np.ndarray = None  # (ncells, 8) topological vertex ids
+    style: str = "consecutive"
     thin_wall_faces: List[dict] = dataclasses.field(default_factory=list)
 
     @property
@@ -76,9 +78,13 @@ class Spaces:
         return self.mesh.ncells
 
     @property
+    def field_order(self):
+        return ("u", "j", "p", "phi") if self.style == "badia2024" else ("u", "p", "j", "phi")
+
+    @property
     def offsets(self) -> Dict[str, int]:
         o, out = 0, {}
-        for f in ("u", "p", "j", "phi"):
+        for f in self.field_order:
             out[f] = o
             o += self.nfree[f]
         return out
@@ -92,10 +98,13 @@ class Spaces:
         return {f: slice(off[f], off[f] + self.nfree[f]) for f in ("u", "p", "j", "phi")}
 
 
-def setup_fe_spaces(mesh: CartesianMesh, u_tags, j_tags, thin_walls: Sequence[dict] = ()) -> Spaces:
+def setup_fe_spaces(mesh: CartesianMesh, u_tags, j_tags, thin_walls: Sequence[dict] = (),
+                    style: str = "consecutive") -> Spaces:
     """Build the dof tables.  ``u_tags``/``j_tags`` are Dirichlet tag names (or
     box entity ids); ``thin_walls`` is a list of dicts with ``domain`` (tags),
-    ``cw``, ``tau``, ``jw`` (src/parameters.jl:938-947)."""
+    ``cw``, ``tau``, ``jw`` (src/parameters.jl:938-947).  ``style`` is the
+    multifield numbering (src/fespaces.jl:4-9): "consecutive" = [u; p; j; phi],
+    "badia2024" = [u; j], p, phi."""
     nx, ny, nz = mesh.nc
     ncells = mesh.ncells
     LX, LY, LZ = mesh.lattice_shape()
@@ -168,9 +177,16 @@ def setup_fe_spaces(mesh: CartesianMesh, u_tags, j_tags, thin_walls: Sequence[di
 
     # ---- assemble the multifield table --------------------------------------
     n_p, n_phi = 4 * ncells, 8 * ncells
-    off_u, off_p = 0, n_u_free
-    off_j = off_p + n_p
-    off_phi = off_j + n_j_free
+    if style == "badia2024":
+        off_u, off_j = 0, n_u_free
+        off_p = off_j + n_j_free
+        off_phi = off_p + n_p
+    elif style == "consecutive":
+        off_u, off_p = 0, n_u_free
+        off_j = off_p + n_p
+        off_phi = off_j + n_j_free
+    else:
+        raise ValueError(f"unknown multifield style {style!r}")
     cd = np.empty((ncells, NDOF), dtype=np.int64)
     nd = u_node_dof[u_node_rank]  # (ncells, 27)
     for c in range(3):
@@ -202,6 +218,8 @@ def setup_fe_spaces(mesh: CartesianMesh, u_tags, j_tags, thin_walls: Sequence[di
         face_ids=face_id,
         face_dirichlet=j_dir_face,
         face_dof=face_dof,
+        cell_verts=vert_id.astype(np.int32),
+        style=style,
     )
     # ---- thin-wall boundary faces -------------------------------------------
     for tw in thin_walls:

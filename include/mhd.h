@@ -40,6 +40,10 @@ typedef struct {
   int32_t ncells;
   int32_t nfaces_tw;            /* thin-wall boundary faces */
   const double *cell_xyz;       /* [ncells][8][3] */
+  const int32_t *cell_verts;    /* [ncells][8] topological vertex ids (periodic images identified); NULL
+                                   disables the vertex-star smoother of mhd_newton_solve */
+  int32_t nverts;
+  int32_t pad_;
   const int32_t *tw_face_cell;  /* [nfaces_tw] owning cell */
   const int8_t *tw_face_lface;  /* [nfaces_tw] local face 0..5 (z-, z+, y-, y+, x-, x+) */
   const double *tw_tau;         /* [nfaces_tw] penalty tau */
@@ -48,11 +52,15 @@ typedef struct {
 } mhd_mesh;
 
 /* Cell dof tables of the multifield space.  Replaces setup_fe_spaces
- * (src/fespaces.jl:13-46): Consecutive [u; p; j; phi] free numbering. */
+ * (src/fespaces.jl:13-46).  block_style selects the multifield numbering
+ * (src/fespaces.jl:4-9): 0 = Consecutive [u; p; j; phi] (the :julia solver),
+ * 1 = Badia2024 blocks [u; j], p, phi, i.e. free ids ordered u, j, p, phi.
+ * mhd_newton_solve needs block_style 1 and the p / phi dofs of every cell
+ * consecutive (cell-major L2 spaces). */
 typedef struct {
   int32_t nfree[4];             /* free dofs of u, p, j, phi */
   int32_t ndir;                 /* Dirichlet values */
-  int32_t pad_;
+  int32_t block_style;
   const int32_t *cell_dofs;     /* [ncells][129]: >= 0 free id, < 0 -> Dirichlet value -(id+1) */
   const int8_t *rt_sign;        /* [ncells][6] face orientation signs, NULL = all +1 */
   const double *dir_vals;       /* [ndir] */

@@ -36,18 +36,19 @@ def cfd_physics(Ha, Re, B=(0.0, 1.0, 0.0), f=(0.0, 0.0, 1.0), convection="newton
                 f0=np.array(f, float), convection=convection, zeta_u=zeta_u, zeta_j=zeta_j)
 
 
-def hunt_case(cx, Ha=50.0, nz=3, BL_adapted=False, domain=(-1.0, 1.0, -1.0, 1.0, 0.0, 0.1), convection="newton"):
+def hunt_case(cx, Ha=50.0, nz=3, BL_adapted=False, domain=(-1.0, 1.0, -1.0, 1.0, 0.0, 0.1), convection="newton",
+              style="consecutive"):
     """Hunt flow: noslip walls, insulating side walls, B along y, f = e_z."""
     mesh = g.meshers.hunt_generate_base_mesh((cx, cx), Ha=Ha, BL_adapted=BL_adapted, nz=nz, domain=domain)
-    sp = g.fespaces.setup_fe_spaces(mesh, "noslip", "insulating")
+    sp = g.fespaces.setup_fe_spaces(mesh, "noslip", "insulating", style=style)
     return sp, cfd_physics(Ha, 1.0, convection=convection)
 
 
 def duct_thin_wall_case(nc, Ha, Re, cw, tau=100.0, domain=(-1.0, 1.0, -1.0, 1.0, 0.0, 0.3),
-                        convection="newton", map=None, zeta_u=0.0, zeta_j=0.0):
+                        convection="newton", map=None, zeta_u=0.0, zeta_j=0.0, style="consecutive"):
     """Duct with thin conducting walls on all four walls (no j Dirichlet)."""
     mesh = g.meshers.hunt_generate_base_mesh(nc[:2], nz=nc[2], domain=domain, Ha=Ha, map=map or (lambda x: x))
-    sp = g.fespaces.setup_fe_spaces(mesh, "noslip", [], [dict(domain="walls", cw=cw, tau=tau, jw=0.0)])
+    sp = g.fespaces.setup_fe_spaces(mesh, "noslip", [], [dict(domain="walls", cw=cw, tau=tau, jw=0.0)], style=style)
     return sp, cfd_physics(Ha, Re, convection=convection, zeta_u=zeta_u, zeta_j=zeta_j)
 
 
@@ -90,3 +91,22 @@ def manufactured_case(nc=(3, 3, 3), B=(0.0, 1.0, 0.0), alpha=1.0, beta=1.0, gamm
     phys = dict(alpha=alpha, beta=beta, gamma=gamma, sigma=sigma, B=Bv, f0=f0, fx=F, g0=g0, gx=G,
                 convection=convection)
     return sp, phys, xs, dv
+
+
+def field_errors(sp, x, xref):
+    """Relative errors per field; the pressure is compared modulo its constant
+    (the constant of dP1 is the all-ones coefficient vector of every cell)."""
+    out = {}
+    for f, s in sp.field_slices().items():
+        a, b = x[s].copy(), xref[s].copy()
+        if f == "p":
+            a -= a.mean()
+            b -= b.mean()
+        nb = max(np.linalg.norm(b), 1e-6 * np.linalg.norm(xref))
+        out[f] = np.linalg.norm(a - b) / nb
+    xa, xb = x.copy(), xref.copy()
+    ps = sp.field_slices()["p"]
+    xa[ps] -= xa[ps].mean()
+    xb[ps] -= xb[ps].mean()
+    out["all"] = np.linalg.norm(xa - xb) / np.linalg.norm(xb)
+    return out
