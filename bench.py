@@ -1,0 +1,277 @@
+#!/usr/bin/env python
+"""Benchmark of the GridapMHD hot path on one MI355X.
+
+Workload (the GPU drop-in configuration): fully developed duct
+[-1,1]^2 x [0,0.3], uniform 64 x 64 x 8 hexahedra, periodic in z, Q2 / dP1 /
+RT1 / dQ1 spaces (1,957,936 free dofs with thin walls), Ha = 100, Re = 10
+(:cfd scaling: alpha = 1, beta = 0.1, gamma = 1000), B = (0,1,0), f = e_z,
+Newton convection, thin conducting walls (cw = 0.01, tau = 100) on all four
+walls.  Synthetic inputs only.
+
+A "step" is one Newton step: residual assembly + Jacobian assembly +
+preconditioner set-up + block-preconditioned FGMRES solve + update, from the
+zero initial guess.  `value` is the mixed-precision Newton-step time; the FP64
+form is reported beside it, both with their relative solution error against a
+tightly converged FP64 solve of the same problem.  Assembly throughput
+(cells/s) is timed with HIP events on the stream the kernels run on, with
+inputs resident in HBM.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Newton-step s (asm+solve, 1.5M DOF) mixed vs FP64 at 1e-8 rel err; cells/s asm"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def splitmix64_uniform(n: int, seed: int) -> np.ndarray:
+    """U[0,1) doubles from SplitMix64, vectorised (wrapping uint64 arithmetic)."""
+    with np.errstate(over="ignore"):
+        s = np.uint64(seed) + (np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+        z = s
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+CONFIGS = {
+    "c2": dict(nc=(64, 64, 8), Ha=100.0, Re=10.0, cw=0.01),
+    "c1": dict(nc=(8, 8, 3), Ha=10.0, Re=1.0, cw=0.01),
+    "mid": dict(nc=(32, 32, 8), Ha=100.0, Re=10.0, cw=0.01),
+}
+
+
+def build_problem(cfg):
+    import gridapmhd_amd as g
+    nc = cfg["nc"]
+    mesh = g.meshers.hunt_generate_base_mesh(nc[:2], nz=nc[2], domain=(-1.0, 1.0, -1.0, 1.0, 0.0, 0.3),
+                                             Ha=cfg["Ha"], map=lambda x: x)
+    sp = g.fespaces.setup_fe_spaces(mesh, "noslip", [], [dict(domain="walls", cw=cfg["cw"], tau=100.0, jw=0.0)],
+                                    style="badia2024")
+    Ha, Re = cfg["Ha"], cfg["Re"]
+    phys = dict(alpha=1.0, beta=1.0 / Re, gamma=Ha * Ha / Re, sigma=1.0, B=np.array([0.0, 1.0, 0.0]),
+                f0=np.array([0.0, 0.0, 1.0]), convection="newton", zeta_u=1.0, zeta_j=1.0)
+    return sp, phys
+
+
+def time_events(fn, reps, stream):
+    import torch
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record(stream)
+    for _ in range(reps):
+        fn()
+    e.record(stream)
+    e.synchronize()
+    return s.elapsed_time(e) / 1e3 / reps
+
+
+def cpu_baseline(cfg, sp, phys, budget_s=12.0):
+    """Oracle (C restatement, OpenMP on all host cores) Jacobian assembly on a
+    bounded sample of the same C2 cells."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cases
+    import oracle
+    oracle.build()
+    pb = cases.problem_dict(sp, dict(phys))
+    orc = oracle.Oracle(pb)
+    x = splitmix64_uniform(sp.ndofs, 1234)
+    n = 64
+    t = time.perf_counter()
+    orc.cell_jacobians(x, 0, n)
+    dt = time.perf_counter() - t
+    n2 = int(min(sp.ncells, max(n, n * budget_s / max(dt, 1e-6))))
+    t = time.perf_counter()
+    orc.cell_jacobians(x, 0, n2)
+    dt2 = time.perf_counter() - t
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        model = platform.processor()
+    return dict(value=n2 / dt2, unit="cells/s (Jacobian element matrices)", cores=cores, kind="port",
+                sample=f"oracle cell Jacobians of the first {n2} of {sp.ncells} cells of the same duct, "
+                       f"{dt2:.1f} s on {cores} threads ({model})")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed Newton steps per precision")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--asm-reps", type=int, default=10)
+    ap.add_argument("--mixed", type=int, default=1, choices=[1, 2], help="mixed mode: 1 FP32 / 2 BF16 patch blocks")
+    ap.add_argument("--restart", type=int, default=100)
+    ap.add_argument("--inner", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import gridapmhd_amd as g
+
+    cfg = CONFIGS[args.config]
+    t0 = time.perf_counter()
+    sp, phys = build_problem(cfg)
+    log(f"mesh+spaces {cfg['nc']}: ndofs={sp.ndofs} ({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    ctx = g._lib.context_from_spaces(sp, phys)
+    t_ctx = time.perf_counter() - t0
+    log(f"context: nnz={ctx.nnz} ({t_ctx:.1f}s)")
+    dev = torch.device("cuda:0")
+    # a dedicated (default-priority) stream: the library launches on it and the
+    # events below are recorded on it, so they bracket exactly the kernels
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
+    sh = stream.cuda_stream
+    assert sh != 0
+    x = torch.from_numpy(splitmix64_uniform(sp.ndofs, 1234)).to(dev)
+    val = torch.empty(ctx.nnz, dtype=torch.float64, device=dev)
+    r = torch.empty(sp.ndofs, dtype=torch.float64, device=dev)
+    ncells = sp.ncells
+
+    # ---- assembly throughput -----------------------------------------------
+    for _ in range(2):
+        ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh)
+        ctx.residual_d(x.data_ptr(), r.data_ptr(), sh)
+    torch.cuda.synchronize()
+    t_jac = time_events(lambda: ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh), args.asm_reps, stream)
+    t_res = time_events(lambda: ctx.residual_d(x.data_ptr(), r.data_ptr(), sh), args.asm_reps, stream)
+    log(f"jacobian {t_jac * 1e3:.3f} ms ({ncells / t_jac:.3e} cells/s), residual {t_res * 1e3:.3f} ms")
+    nnz = ctx.nnz
+    jac_bytes = nnz * 8 + ncells * (129 * (8 + 4) + 24 * 8)
+    # ---- SpMV --------------------------------------------------------------
+    y = torch.empty_like(r)
+    ctx.spmv_d(x.data_ptr(), y.data_ptr(), sh)
+    t_spmv = time_events(lambda: ctx.spmv_d(x.data_ptr(), y.data_ptr(), sh), 20, stream)
+    spmv_bytes = nnz * (8 + 4) + (sp.ndofs + 1) * 8 + 2 * sp.ndofs * 8
+    log(f"spmv {t_spmv * 1e3:.3f} ms ({spmv_bytes / t_spmv / 1e9:.0f} GB/s)")
+
+    # ---- solves ------------------------------------------------------------
+    x0 = np.zeros(sp.ndofs)
+
+    def opts(prec, rtol, ls_rtol, maxk):
+        return ctx.default_opts(precision=prec, rtol=rtol, atol=1e-14, ls_rtol=ls_rtol, max_krylov=maxk,
+                                restart=args.restart, inner_iters=args.inner, max_newton=8)
+
+    t0 = time.perf_counter()
+    x_ref, info_ref, rc_ref = ctx.newton_solve(x0, opts(0, 1e-11, 1e-11, 3000))
+    t_ref = time.perf_counter() - t0
+    log(f"reference FP64 solve: newton={info_ref.newton_iters} krylov={info_ref.krylov_iters} "
+        f"|R|={info_ref.residual:.2e} rc={rc_ref} ({t_ref:.1f}s)")
+
+    def rel_err(xs):
+        ps = sp.field_slices()["p"]
+        a, b = xs.copy(), x_ref.copy()
+        a[ps] -= a[ps].mean()
+        b[ps] -= b[ps].mean()
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    results = {}
+    for label, prec in (("fp64", 0), ("mixed", args.mixed)):
+        o = opts(prec, 1e-8, 1e-9, 3000)
+        for _ in range(args.warmup):
+            ctx.newton_solve(x0, o)
+        t_tot, n_tot, k_tot = 0.0, 0, 0
+        parts = np.zeros(4)
+        for _ in range(args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xs, info, rc = ctx.newton_solve(x0, o)
+            torch.cuda.synchronize()
+            t_tot += time.perf_counter() - t0
+            n_tot += info.newton_iters
+            k_tot += info.krylov_iters
+            parts += [info.t_residual, info.t_jacobian, info.t_setup, info.t_solve]
+        err = rel_err(xs)
+        results[label] = dict(newton_step_s=t_tot / max(n_tot, 1), newton_iters=n_tot / args.steps,
+                              krylov_iters=k_tot / args.steps, rel_err=err, converged=bool(info.converged),
+                              phase_s=dict(zip(("residual", "jacobian", "setup", "solve"),
+                                               (parts / max(n_tot, 1)).round(4).tolist())))
+        log(f"{label}: {results[label]}")
+
+    # ---- smoother kernel (vertex-star additive Schwarz, FP64 blocks) --------
+    ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh)
+    torch.cuda.synchronize()
+    ctx.precond_setup(opts(0, 1e-8, 1e-9, 10))
+    npatch, pdim, pent = ctx.smoother_info()
+    n1 = sp.nfree["u"] + sp.nfree["j"]
+    rr = torch.rand(n1, dtype=torch.float64, device=dev)
+    yy = torch.zeros(n1, dtype=torch.float64, device=dev)
+    ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh)
+    t_sm = time_events(lambda: ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh), 10, stream)
+    sm_bytes = pent * 8
+    log(f"smoother apply {t_sm * 1e3:.3f} ms ({sm_bytes / t_sm / 1e9:.0f} GB/s), npatch={npatch} dim={pdim}")
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle) ...")
+        cpu = cpu_baseline(cfg, sp, phys)
+        log(f"cpu: {cpu}")
+
+    def roof(name, nbytes, t):
+        a = nbytes / t / 1e9
+        return dict(kernel=name, bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(a / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=int(nbytes),
+                    avg_launch_ms=round(t * 1e3, 4))
+
+    mixed, fp64 = results["mixed"], results["fp64"]
+    out = {
+        "metric": METRIC,
+        "value": round(mixed["newton_step_s"], 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["newton_step_s"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 assembly/refinement + f32 inner" + (" + bf16 smoother" if args.mixed == 2 else ""),
+        "data": "synthetic",
+        "config": {"workload": f"thin-wall duct {'x'.join(map(str, cfg['nc']))} hex, Q2/dP1/RT1/dQ1, "
+                               f"Ha={cfg['Ha']:g}, Re={cfg['Re']:g}, cw={cfg['cw']:g}, Newton convection",
+                   "ndofs": sp.ndofs, "ncells": ncells, "nnz": nnz, "zeta": [1.0, 1.0],
+                   "restart": args.restart, "inner_iters": args.inner},
+        "fp64_newton_step_s": round(fp64["newton_step_s"], 4),
+        "mixed_over_fp64_speedup": round(fp64["newton_step_s"] / mixed["newton_step_s"], 3),
+        "rel_err": {"fp64": fp64["rel_err"], "mixed": mixed["rel_err"]},
+        "solves": results,
+        "asm_cells_per_s": round(ncells / t_jac, 1),
+        "asm_jacobian_ms": round(t_jac * 1e3, 4),
+        "asm_residual_ms": round(t_res * 1e3, 4),
+        "residual_cells_per_s": round(ncells / t_res, 1),
+        "roofline": roof("jacobian assembly (memset + k_cell_jacobian + k_face_jacobian)", jac_bytes, t_jac),
+        "roofline_kernels": [roof("k_spmv_seg/k_spmv fp64 CSR", spmv_bytes, t_spmv),
+                             roof("k_patch_apply<double> vertex-star Schwarz", sm_bytes, t_sm)],
+        "reference_solve": dict(seconds=round(t_ref, 2), newton=info_ref.newton_iters,
+                                krylov=info_ref.krylov_iters, residual=info_ref.residual),
+        "context_setup_s": round(t_ctx, 2),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

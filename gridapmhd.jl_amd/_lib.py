@@ -68,6 +68,10 @@ EXPORTS = {
     "mhd_newton_solve": (ctypes.c_int, [c_void_p, c_void_p, ctypes.POINTER(MhdSolverOpts),
                                         ctypes.POINTER(MhdSolveInfo)]),
     "mhd_default_solver_opts": (None, [ctypes.POINTER(MhdSolverOpts)]),
+    "mhd_precond_setup": (ctypes.c_int, [c_void_p, ctypes.POINTER(MhdSolverOpts)]),
+    "mhd_smoother_apply_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhd_smoother_info": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
+                                         ctypes.POINTER(c_int64)]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -207,6 +211,25 @@ class Context:
 
     def spmv_d(self, d_x: int, d_y: int, stream: int = 0):
         return self._check(self.lib.mhd_spmv_d(self.h, d_x, d_y, stream or None), "mhd_spmv_d")
+
+    def default_opts(self, **kw) -> MhdSolverOpts:
+        o = MhdSolverOpts()
+        self.lib.mhd_default_solver_opts(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def precond_setup(self, opts: MhdSolverOpts):
+        return self._check(self.lib.mhd_precond_setup(self.h, ctypes.byref(opts)), "mhd_precond_setup")
+
+    def smoother_apply_d(self, d_r: int, d_y: int, stream: int = 0):
+        return self._check(self.lib.mhd_smoother_apply_d(self.h, d_r, d_y, stream or None), "mhd_smoother_apply_d")
+
+    def smoother_info(self):
+        a, b, c_ = c_int64(), c_int64(), c_int64()
+        self._check(self.lib.mhd_smoother_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c_)),
+                    "mhd_smoother_info")
+        return a.value, b.value, c_.value
 
     def newton_solve(self, x0, opts: Optional[MhdSolverOpts] = None):
         x = np.array(x0, dtype=np.float64, copy=True)

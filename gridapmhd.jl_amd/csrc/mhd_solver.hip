@@ -16,11 +16,11 @@
 // patch of a vertex holds every u / j dof whose support lies in the cells
 // around it); the dense patch inverses come from batched LU.
 //
-// Precision: precision 0 runs everything in FP64.  Precision 1 keeps FP64 for
-// the residual assembly, the outer FGMRES (the refinement residual, its
-// classical Gram-Schmidt with reorthogonalisation, the Newton update) and the
-// p / phi Schur solves, and runs the inner A11 GMRES in FP32 with an FP32 copy
-// of A11 and BF16 patch inverses.
+// Precision: precision 0 runs everything in FP64.  Precisions 1 and 2 keep
+// FP64 for the residual assembly, the outer FGMRES (the refinement residual,
+// its classical Gram-Schmidt with reorthogonalisation, the Newton update) and
+// the p / phi Schur solves, and run the inner A11 GMRES in FP32 with an FP32
+// copy of A11; the patch inverses are stored in FP32 (1) or BF16 (2).
 #include <hip/hip_bf16.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -472,17 +472,17 @@ int inner_alloc(mhd_ctx *c, int prec) {
   void *ptrs[] = {S->V1, S->Z1, S->w1, S->x1, S->r1};
   for (void *p : ptrs)
     if (p) MHD_HIP(hipFree(p));
-  const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
+  const size_t es = prec == 0 ? sizeof(double) : sizeof(float);  // inner vectors
   const size_t n1 = (size_t)S->n1;
   MHD_HIP(hipMalloc(&S->V1, es * (S->kin + 1) * n1 + es));
   MHD_HIP(hipMalloc(&S->Z1, es * std::max(S->kin, 1) * n1 + es));
   MHD_HIP(hipMalloc(&S->w1, es * n1 + es));
   MHD_HIP(hipMalloc(&S->x1, es * n1 + es));
   MHD_HIP(hipMalloc(&S->r1, es * n1 + es));
-  if (prec == 1) {
-    if (!S->d_val32) MHD_HIP(hipMalloc(&S->d_val32, sizeof(float) * std::max<int64_t>(c->nnz, 1)));
-    if (!S->d_pA16) MHD_HIP(hipMalloc(&S->d_pA16, sizeof(__hip_bfloat16) * (size_t)S->npatch * S->PS * S->PS));
-  }
+  if (prec >= 1 && !S->d_val32) MHD_HIP(hipMalloc(&S->d_val32, sizeof(float) * std::max<int64_t>(c->nnz, 1)));
+  if (prec == 1 && !S->d_pA32) MHD_HIP(hipMalloc(&S->d_pA32, sizeof(float) * (size_t)S->npatch * S->PS * S->PS));
+  if (prec == 2 && !S->d_pA16)
+    MHD_HIP(hipMalloc(&S->d_pA16, sizeof(__hip_bfloat16) * (size_t)S->npatch * S->PS * S->PS));
   S->inner_prec = prec;
   return 0;
 }
@@ -572,6 +572,9 @@ int patch_as(mhd_ctx *c, const T *r, T *y, int prec) {
   if (prec == 0)
     hipLaunchKernelGGL((k_patch_apply<double, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch, S->PS,
                        S->d_psize, S->d_pidx, S->d_pA, r, y);
+  else if (prec == 1)
+    hipLaunchKernelGGL((k_patch_apply<float, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch, S->PS,
+                       S->d_psize, S->d_pidx, S->d_pA32, r, y);
   else
     hipLaunchKernelGGL((k_patch_apply<__hip_bfloat16, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch,
                        S->PS, S->d_psize, S->d_pidx, S->d_pA16, r, y);
@@ -649,7 +652,7 @@ int precond(mhd_ctx *c, const double *v, double *z, int prec) {
     hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, S->t1,
                        (float *)S->r1);
     MHD_HIP(hipGetLastError());
-    if ((rc = inner_solve<float>(c, 1))) return rc;
+    if ((rc = inner_solve<float>(c, prec))) return rc;
     hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1,
                        (const float *)S->x1, z);
   }
@@ -673,7 +676,7 @@ int precond_setup(mhd_ctx *c, int prec) {
     S->alpha_p = ap;
     S->alpha_phi = aphi;
   }
-  if (prec == 1)
+  if (prec >= 1)
     hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(c->nnz)), dim3(TB), 0, c->stream, c->nnz, c->d_val,
                        S->d_val32);
   const size_t pelems = (size_t)S->npatch * S->PS * S->PS;
@@ -691,6 +694,9 @@ int precond_setup(mhd_ctx *c, int prec) {
     return -2;
   }
   if (prec == 1)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks((int64_t)pelems)), dim3(TB), 0, c->stream,
+                       (int64_t)pelems, S->d_pA, S->d_pA32);
+  if (prec == 2)
     hipLaunchKernelGGL(k_to_bf16, dim3(nblocks((int64_t)pelems)), dim3(TB), 0, c->stream, (int64_t)pelems, S->d_pA,
                        S->d_pA16);
   MHD_HIP(hipGetLastError());
@@ -799,11 +805,57 @@ void mhd_default_solver_opts(mhd_solver_opts *o) {
   o->inner_iters = 8;
 }
 
+int mhd_precond_setup(mhd_ctx *c, const mhd_solver_opts *o) {
+  if (!c || !o) return -1;
+  if (o->precision < 0 || o->precision > 2) {
+    c->err = "precision must be 0, 1 or 2";
+    return -1;
+  }
+  MHD_HIP(hipSetDevice(c->device));
+  int rc;
+  if (c->solver && (c->solver->m != std::max(2, std::min(o->restart, 120)) ||
+                    c->solver->kin != std::max(0, std::min(o->inner_iters, 120))))
+    mhd_solver_free(c);
+  if ((rc = solver_init(c, o))) return rc;
+  if ((rc = precond_setup(c, o->precision))) return rc;
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mhd_smoother_apply_d(mhd_ctx *c, const double *d_r, double *d_y, void *stream) {
+  if (!c || !d_r || !d_y) return -1;
+  if (!c->solver || c->solver->inner_prec != 0) {
+    c->err = "mhd_smoother_apply_d needs mhd_precond_setup with precision 0 first";
+    return -1;
+  }
+  MHD_HIP(hipSetDevice(c->device));
+  Solver *S = c->solver;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipLaunchKernelGGL((k_patch_apply<double, double>), dim3(S->npatch), dim3(256), 0, s, S->npatch, S->PS, S->d_psize,
+                     S->d_pidx, S->d_pA, d_r, d_y);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+int mhd_smoother_info(mhd_ctx *c, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries) {
+  if (!c || !c->solver) return -1;
+  if (npatch) *npatch = c->solver->npatch;
+  if (patch_dim) *patch_dim = c->solver->PS;
+  if (patch_entries) {  // sum over patches of size^2 (entries read per application)
+    std::vector<int32_t> ps(c->solver->npatch);
+    MHD_HIP(hipMemcpy(ps.data(), c->solver->d_psize, sizeof(int32_t) * ps.size(), hipMemcpyDeviceToHost));
+    int64_t t = 0;
+    for (int32_t v : ps) t += (int64_t)v * v;
+    *patch_entries = t;
+  }
+  return 0;
+}
+
 int mhd_newton_solve(mhd_ctx *c, double *x_host, const mhd_solver_opts *o, mhd_solve_info *info) {
   if (!c || !x_host || !o || !info) return -1;
   std::memset(info, 0, sizeof(*info));
-  if (o->precision != 0 && o->precision != 1) {
-    c->err = "precision must be 0 (FP64) or 1 (mixed)";
+  if (o->precision < 0 || o->precision > 2) {
+    c->err = "precision must be 0 (FP64), 1 (FP32 inner, FP32 patches) or 2 (FP32 inner, BF16 patches)";
     return -1;
   }
   MHD_HIP(hipSetDevice(c->device));

@@ -85,7 +85,8 @@ typedef struct {
   int32_t max_newton;           /* :niter (20) */
   int32_t restart;              /* FGMRES Krylov dimension m (:niter_ls, 15) */
   int32_t max_krylov;           /* FGMRES iteration cap per Newton step */
-  int32_t precision;            /* 0 FP64, 1 mixed (FP32 inner / BF16 smoother blocks, FP64 refinement) */
+  int32_t precision;            /* 0 FP64; 1 mixed: FP32 inner GMRES + FP32 smoother blocks inside FP64
+                                   FGMRES refinement; 2 as 1 with BF16 smoother blocks */
   double rtol, atol;            /* Newton: ||R|| <= max(atol, rtol ||R0||) */
   double ls_rtol;               /* linear: ||r|| <= ls_rtol ||b|| */
   int32_t inner_iters;          /* smoothing sweeps of the [u,j] block solve */
@@ -144,6 +145,19 @@ int mhd_spmv_d(mhd_ctx *ctx, const double *d_x, double *d_y, void *stream);
  * x_free is the initial guess on entry, the solution on return. */
 int mhd_newton_solve(mhd_ctx *ctx, double *x_free, const mhd_solver_opts *opts, mhd_solve_info *info);
 void mhd_default_solver_opts(mhd_solver_opts *opts);
+
+/* Preconditioner set-up for the last assembled Jacobian: the numerical_setup!
+ * of the block solver (src/Solvers/badia2024.jl:20-31) -- Schur scalings,
+ * vertex-star patch extraction and batched LU.  opts->precision selects the
+ * storage of the patch inverses (0 FP64, 1 FP32, 2 BF16). */
+int mhd_precond_setup(mhd_ctx *ctx, const mhd_solver_opts *opts);
+
+/* y += sum_P R_P^T A_P^-1 R_P r on the [u,j] block (device pointers, length
+ * nfree[u] + nfree[j]; y is accumulated into).  The smoother of the [u,j]
+ * block solve; needs mhd_precond_setup with precision 0. */
+int mhd_smoother_apply_d(mhd_ctx *ctx, const double *d_r, double *d_y, void *stream);
+/* Patch count, padded patch dimension, sum over patches of size^2. */
+int mhd_smoother_info(mhd_ctx *ctx, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries);
 
 #ifdef __cplusplus
 }
