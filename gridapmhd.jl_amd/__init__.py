@@ -1,2 +1,13 @@
-"""MI355X-native backend for GridapMHD's H1-HDiv assembly + block-preconditioned Newton-Krylov hot path."""
-from . import meshers, fespaces, _lib  # noqa: F401
+"""MI355X-native backend for GridapMHD's H1-HDiv assembly + block-preconditioned Newton-Krylov hot path.
+
+Public surface (mirrors the reference's entry points):
+  main(params) -> (xh, full_params, output)        src/main.jl:122-169
+  add_default_params(params)                        src/parameters.jl:85-137
+  hunt(...), thin_wall_duct(...)                    src/Applications/hunt.jl, FullyDevelopedFlow.jl
+  meshers.hunt_generate_base_mesh, fespaces.setup_fe_spaces
+  _lib.Context                                      ctypes binding of include/mhd.h
+"""
+from . import meshers, fespaces, _lib, parameters  # noqa: F401
+from .parameters import add_default_params, default_solver_params  # noqa: F401
+from .main import main, MultiFieldFEFunction  # noqa: F401
+from .applications import hunt, thin_wall_duct  # noqa: F401

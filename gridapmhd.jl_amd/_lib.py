@@ -61,6 +61,7 @@ EXPORTS = {
     "mhd_pattern": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "mhd_residual": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "mhd_jacobian": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "mhd_cell_integrals": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "mhd_residual_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhd_jacobian_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhd_csr_arrays": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]),
@@ -158,6 +159,7 @@ class Context:
         n, nnz = c_int64(), c_int64()
         self._check(self.lib.mhd_sizes(self.h, ctypes.byref(n), ctypes.byref(nnz)), "mhd_sizes")
         self.n, self.nnz = n.value, nnz.value
+        self.ncells = int(dofs.shape[0])
 
     def _check(self, rc, what):
         if rc < 0:
@@ -193,6 +195,13 @@ class Context:
         nz = np.empty(self.nnz)
         self._check(self.lib.mhd_jacobian(self.h, _ptr(x), _ptr(nz)), "mhd_jacobian")
         return nz
+
+    def cell_integrals(self, x):
+        """(ncells, 6): int u.u, int grad u:grad u, int j.j, int (div j)^2, int p, |K|."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty((self.ncells, 6))
+        self._check(self.lib.mhd_cell_integrals(self.h, _ptr(x), _ptr(out)), "mhd_cell_integrals")
+        return out
 
     def set_params(self, phys):
         self.params = make_params(phys)

@@ -222,6 +222,21 @@ int mhd_spmv_d(mhd_ctx *c, const double *d_x, double *d_y, void *stream) {
   return mhd_launch_spmv(c, c->d_val, d_x, d_y, stream ? (hipStream_t)stream : c->stream);
 }
 
+int mhd_cell_integrals(mhd_ctx *c, const double *x, double *out) {
+  if (!c || !x || !out) return -1;
+  MHD_HIP(hipSetDevice(c->device));
+  double *d_out = nullptr;
+  MHD_HIP(hipMalloc(&d_out, sizeof(double) * 6 * (size_t)c->ncells));
+  MHD_HIP(hipMemcpyAsync(c->d_x, x, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+  int rc = mhd_launch_cell_integrals(c, c->d_x, d_out, c->stream);
+  if (!rc) {
+    MHD_HIP(hipMemcpyAsync(out, d_out, sizeof(double) * 6 * (size_t)c->ncells, hipMemcpyDeviceToHost, c->stream));
+    MHD_HIP(hipStreamSynchronize(c->stream));
+  }
+  MHD_HIP(hipFree(d_out));
+  return rc;
+}
+
 int mhd_residual(mhd_ctx *c, const double *x, double *r) {
   if (!c || !x || !r) return -1;
   MHD_HIP(hipSetDevice(c->device));

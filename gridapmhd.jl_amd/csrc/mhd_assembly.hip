@@ -30,17 +30,25 @@ struct CellLDS {
     };
     struct {
       double D[4][NU];       // AL-u: int psi_k div v_b
+      double MD[4][NU];      // AL-u: M^-1 D
       double Mi[16];         // AL-u: inverse dP1 mass
       double cu[4];          // AL-u: M^-1 int psi div u
+      double Mq[16];         // AL-u: dP1 mass
+      double dq[4];          // AL-u: int psi div u
     };
   };
   double idet[NQ];           // 1 / det J
   double xq[NQ][3];          // physical point
+  double q2[NQ][27];         // reference Q2 values
+  double p1[NQ][4];          // reference dP1 values
+  double q1[NQ][8];          // reference dQ1 values
   double gphi[NQ][27][3];    // physical Q2 gradients
+  double cphi[NQ][27];       // u . grad phi (convective derivative)
   double psi[NQ][36][3];     // Piola-mapped RT basis (with face signs)
   double dpsi[NQ][36];       // its divergence
   double u[NQ][3];           // state of x
   double gu[NQ][9];          // grad u [i][k] = d u_k / d x_i
+  double wN[NQ][9];          // W alpha d_d u_c  at [3c + d]  (Newton term weights)
   double p[NQ], jv[NQ][3], divj[NQ], ph[NQ], divu[NQ];
   double xl[NDOF];
   double xyz[24];
@@ -132,6 +140,9 @@ __device__ void stage_cell(CellLDS &S, int c, const double *__restrict__ xyz, co
     S.idet[q] = id;
     for (int i = 0; i < 3; ++i) S.xq[q][i] = X[i];
   }
+  for (int idx = tid; idx < NQ * 27; idx += WG) S.q2[idx / 27][idx % 27] = T->q2[idx / 27][idx % 27];
+  for (int idx = tid; idx < NQ * 4; idx += WG) S.p1[idx / 4][idx % 4] = T->p1[idx / 4][idx % 4];
+  for (int idx = tid; idx < NQ * 8; idx += WG) S.q1[idx / 8][idx % 8] = T->q1[idx / 8][idx % 8];
   __syncthreads();
   // push-forward of the Q2 gradients and the RT basis
   for (int idx = tid; idx < NQ * 27; idx += WG) {
@@ -154,14 +165,14 @@ __device__ void stage_cell(CellLDS &S, int c, const double *__restrict__ xyz, co
     const int q = idx / 19, k = idx % 19;
     double acc = 0.0;
     if (k < 3) {  // u_k
-      for (int a = 0; a < 27; ++a) acc += S.xl[k * 27 + a] * T->q2[q][a];
+      for (int a = 0; a < 27; ++a) acc += S.xl[k * 27 + a] * S.q2[q][a];
       S.u[q][k] = acc;
     } else if (k < 12) {  // grad u [i][kk]
       const int i = (k - 3) / 3, kk = (k - 3) % 3;
       for (int a = 0; a < 27; ++a) acc += S.xl[kk * 27 + a] * S.gphi[q][a][i];
       S.gu[q][(k - 3)] = acc;
     } else if (k == 12) {
-      for (int m = 0; m < 4; ++m) acc += S.xl[OFF_P + m] * T->p1[q][m];
+      for (int m = 0; m < 4; ++m) acc += S.xl[OFF_P + m] * S.p1[q][m];
       S.p[q] = acc;
     } else if (k < 16) {
       const int kk = k - 13;
@@ -171,7 +182,7 @@ __device__ void stage_cell(CellLDS &S, int c, const double *__restrict__ xyz, co
       for (int r = 0; r < 36; ++r) acc += S.xl[OFF_J + r] * S.dpsi[q][r];
       S.divj[q] = acc;
     } else if (k == 17) {
-      for (int v = 0; v < 8; ++v) acc += S.xl[OFF_PHI + v] * T->q1[q][v];
+      for (int v = 0; v < 8; ++v) acc += S.xl[OFF_PHI + v] * S.q1[q][v];
       S.ph[q] = acc;
     } else {
       for (int a = 0; a < 27; ++a)
@@ -184,28 +195,44 @@ __device__ void stage_cell(CellLDS &S, int c, const double *__restrict__ xyz, co
     for (int idx = tid; idx < 4 * NU; idx += WG) {
       const int k = idx / NU, b = idx % NU, cb = b / 27, nb = b % 27;
       double acc = 0.0;
-      for (int q = 0; q < NQ; ++q) acc += S.W[q] * T->p1[q][k] * S.gphi[q][nb][cb];
+      for (int q = 0; q < NQ; ++q) acc += S.W[q] * S.p1[q][k] * S.gphi[q][nb][cb];
       S.D[k][b] = acc;
     }
-    if (tid == 0) {
-      double M[16];
-      double dv[4];
-      for (int k = 0; k < 4; ++k) {
-        dv[k] = 0.0;
-        for (int l = 0; l < 4; ++l) {
-          double acc = 0.0;
-          for (int q = 0; q < NQ; ++q) acc += S.W[q] * T->p1[q][k] * T->p1[q][l];
-          M[k * 4 + l] = acc;
-        }
-        for (int q = 0; q < NQ; ++q) dv[k] += S.W[q] * T->p1[q][k] * S.divu[q];
-      }
-      inv4(M, S.Mi);
-      for (int k = 0; k < 4; ++k) {
-        double acc = 0.0;
-        for (int l = 0; l < 4; ++l) acc += S.Mi[k * 4 + l] * dv[l];
-        S.cu[k] = acc;
+    if (tid < 20) {
+      double acc = 0.0;
+      if (tid < 16) {
+        const int k = tid >> 2, l = tid & 3;
+        for (int q = 0; q < NQ; ++q) acc += S.W[q] * S.p1[q][k] * S.p1[q][l];
+        S.Mq[tid] = acc;
+      } else {
+        const int k = tid - 16;
+        for (int q = 0; q < NQ; ++q) acc += S.W[q] * S.p1[q][k] * S.divu[q];
+        S.dq[k] = acc;
       }
     }
+    __syncthreads();
+    if (tid == 0) {
+      inv4(S.Mq, S.Mi);
+      for (int k = 0; k < 4; ++k)
+        S.cu[k] = S.Mi[4 * k + 0] * S.dq[0] + S.Mi[4 * k + 1] * S.dq[1] + S.Mi[4 * k + 2] * S.dq[2] + S.Mi[4 * k + 3] * S.dq[3];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 4 * NU; idx += WG) {
+      const int k = idx / NU, b = idx % NU;
+      double acc = 0.0;
+      for (int l = 0; l < 4; ++l) acc += S.Mi[4 * k + l] * S.D[l][b];
+      S.MD[k][b] = acc;
+    }
+  }
+  __syncthreads();
+  // convective derivative and Newton weights for the Jacobian tiles
+  for (int idx = tid; idx < NQ * 27; idx += WG) {
+    const int q = idx / 27, a = idx % 27;
+    S.cphi[q][a] = S.u[q][0] * S.gphi[q][a][0] + S.u[q][1] * S.gphi[q][a][1] + S.u[q][2] * S.gphi[q][a][2];
+  }
+  for (int idx = tid; idx < NQ * 9; idx += WG) {
+    const int q = idx / 9, cd = idx % 9, cc = cd / 3, dd = cd % 3;
+    S.wN[q][cd] = S.W[q] * ph.alpha * S.gu[q][3 * dd + cc];
   }
   __syncthreads();
 }
@@ -251,9 +278,9 @@ __device__ double jac_entry(const CellLDS &S, const RefTables *__restrict__ T, c
           const double *ga = S.gphi[q][na], *gb = S.gphi[q][nb];
           t = ph.beta * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
           if (ph.convection)
-            t += ph.alpha * T->q2[q][na] * (S.u[q][0] * gb[0] + S.u[q][1] * gb[1] + S.u[q][2] * gb[2]);
+            t += ph.alpha * S.q2[q][na] * (S.u[q][0] * gb[0] + S.u[q][1] * gb[1] + S.u[q][2] * gb[2]);
         }
-        if (ph.convection == 2) t += ph.alpha * T->q2[q][na] * T->q2[q][nb] * S.gu[q][3 * cb + ca];
+        if (ph.convection == 2) t += ph.alpha * S.q2[q][na] * S.q2[q][nb] * S.gu[q][3 * cb + ca];
         acc += S.W[q] * t;
       }
       if (ph.zeta_u != 0.0) {
@@ -267,20 +294,20 @@ __device__ double jac_entry(const CellLDS &S, const RefTables *__restrict__ T, c
       }
     } else if (b < OFF_J) {  // - dp div v
       const int k = b - OFF_P;
-      for (int q = 0; q < NQ; ++q) acc -= S.W[q] * T->p1[q][k] * S.gphi[q][na][ca];
+      for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.p1[q][k] * S.gphi[q][na][ca];
     } else {  // - gamma (dj x B).v
       const int r = b - OFF_J;
       const int i1 = ca == 2 ? 0 : ca + 1, i2 = ca == 0 ? 2 : ca - 1;
       const double b1 = S.B[i1], b2 = S.B[i2];
       for (int q = 0; q < NQ; ++q) {
         const double cr = S.psi[q][r][i1] * b2 - S.psi[q][r][i2] * b1;  // (psi x B)_ca
-        acc -= ph.gamma * S.W[q] * T->q2[q][na] * cr;
+        acc -= ph.gamma * S.W[q] * S.q2[q][na] * cr;
       }
     }
   } else if (a < OFF_J) {  // pressure test: - div du q
     const int k = a - OFF_P;
     const int cb = b / 27, nb = b % 27;
-    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * T->p1[q][k] * S.gphi[q][nb][cb];
+    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.p1[q][k] * S.gphi[q][nb][cb];
   } else if (a < OFF_PHI) {  // current test s = psi_r
     const int r = a - OFF_J;
     if (b < OFF_P) {  // - sigma (du x B).s
@@ -290,7 +317,7 @@ __device__ double jac_entry(const CellLDS &S, const RefTables *__restrict__ T, c
       for (int q = 0; q < NQ; ++q) {
         // (e_cb x B).s = (B x s)_cb
         const double t = b1 * S.psi[q][r][i2] - b2 * S.psi[q][r][i1];
-        acc -= ph.sigma * S.W[q] * T->q2[q][nb] * t;
+        acc -= ph.sigma * S.W[q] * S.q2[q][nb] * t;
       }
     } else if (b < OFF_PHI) {  // dj.s + zeta_j div dj div s
       const int rb = b - OFF_J;
@@ -302,11 +329,11 @@ __device__ double jac_entry(const CellLDS &S, const RefTables *__restrict__ T, c
       }
     } else {  // - sigma dphi div s
       const int v = b - OFF_PHI;
-      for (int q = 0; q < NQ; ++q) acc -= ph.sigma * S.W[q] * T->q1[q][v] * S.dpsi[q][r];
+      for (int q = 0; q < NQ; ++q) acc -= ph.sigma * S.W[q] * S.q1[q][v] * S.dpsi[q][r];
     }
   } else {  // potential test: - div dj phi_t
     const int v = a - OFF_PHI, rb = b - OFF_J;
-    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * T->q1[q][v] * S.dpsi[q][rb];
+    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.q1[q][v] * S.dpsi[q][rb];
   }
   return acc;
 }
@@ -316,9 +343,10 @@ __device__ double res_entry(const CellLDS &S, const RefTables *__restrict__ T, c
   double acc = 0.0;
   if (a < OFF_P) {
     const int ca = a / 27, na = a % 27;
+#pragma unroll 1
     for (int q = 0; q < NQ; ++q) {
       const double *gv = S.gphi[q][na];
-      const double phv = T->q2[q][na];
+      const double phv = S.q2[q][na];
       // beta grad u : grad v with v = phi e_ca -> sum_i gu[i][ca] dphi_i
       double t = ph.beta * (S.gu[q][0 * 3 + ca] * gv[0] + S.gu[q][1 * 3 + ca] * gv[1] + S.gu[q][2 * 3 + ca] * gv[2]);
       if (ph.convection) {
@@ -342,9 +370,11 @@ __device__ double res_entry(const CellLDS &S, const RefTables *__restrict__ T, c
     }
   } else if (a < OFF_J) {
     const int k = a - OFF_P;
-    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.divu[q] * T->p1[q][k];
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.divu[q] * S.p1[q][k];
   } else if (a < OFF_PHI) {
     const int r = a - OFF_J;
+#pragma unroll 1
     for (int q = 0; q < NQ; ++q) {
       const double *s = S.psi[q][r];
       double t = 0.0;
@@ -362,12 +392,160 @@ __device__ double res_entry(const CellLDS &S, const RefTables *__restrict__ T, c
     }
   } else {
     const int v = a - OFF_PHI;
-    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.divj[q] * T->q1[q][v];
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.divj[q] * S.q1[q][v];
   }
   return acc;
 }
 
-__global__ __launch_bounds__(WG) void k_cell_jacobian(int ncells, const double *__restrict__ xyz,
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline d4_t mfma16(double a, double b, d4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Scatter-add one 16x16 MFMA result tile.  Lane l holds rows l/16 + 4 r
+// (r = 0..3), column l%16 of the tile (v_mfma_f64_16x16x4f64 output layout); (row0, col0) are the local dof offsets of the tile,
+// `ent(i, j)` maps tile-local (i, j) to the canonical entry of the rank map.
+template <class EntFn>
+__device__ inline void scatter_tile(const CellLDS &S, const d4_t &acc, double scale, int row0, int nrow, int col0,
+                                    int ncol, const uint16_t *__restrict__ rk, double *__restrict__ val, EntFn ent) {
+  const int lane = threadIdx.x & 63;
+  const int j = lane & 15;
+  if (j >= ncol) return;
+  const int lc = col0 + j;
+  if (S.g[lc] < 0) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = (lane >> 4) + 4 * r;
+    if (i >= nrow) continue;
+    const int lr = row0 + i;
+    if (S.g[lr] < 0) continue;
+    atomicAdd(&val[S.rp[lr] + rk[ent(i, j)]], scale * acc[r]);
+  }
+}
+
+// Jacobian of one cell as FP64 MFMA tiles (16x16x4):
+//   uu(c,d) = delta_cd S + N^{cd} + zeta_u D_c^T M^-1 D_d with
+//     S[a][b]    = sum_q W (beta grad phi_a . grad phi_b + alpha phi_a u.grad phi_b)   (K = 27 q x 4)
+//     N^{cd}[a][b] = sum_q W alpha d_d u_c phi_a phi_b                                (K = 27 q)
+//   X_c[a][r]  = sum_q W phi_a (psi_r x B)_c  ->  uj = -gamma X,  ju = +sigma X^T         (K = 27 q)
+//     (-sigma (e_c phi_a x B).psi_r = +sigma phi_a (psi_r x B)_c, so ju = -(sigma/gamma) uj^T)
+//   jj[r][s]   = sum_q W (psi_r . psi_s + zeta_j div psi_r div psi_s)                    (K = 27 q x 4)
+// The small blocks up / pu (81 x 4) and jphi / phij (36 x 8) run on the vector ALUs.
+__device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const uint16_t *__restrict__ rk,
+                                    double *__restrict__ val) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const d4_t zero = {0.0, 0.0, 0.0, 0.0};
+  // job list: 4 uu super-tiles, 9 jj tiles, 18 cross tiles; wave w takes jobs w, w+4, ...
+  for (int job = wave; job < 4 + 9 + 18; job += WG / 64) {
+    if (job < 4) {
+      const int ti = job >> 1, tj = job & 1;
+      const int a = 16 * ti + li, b = 16 * tj + li;
+      const bool av = a < 27, bv = b < 27;
+      d4_t Sacc = zero;
+#pragma unroll 3
+      for (int q = 0; q < NQ; ++q) {
+        double A = 0.0, Bv = 0.0;
+        if (av) A = lk < 3 ? S.gphi[q][a][lk] : S.q2[q][a];
+        if (bv) Bv = S.W[q] * (lk < 3 ? ph.beta * S.gphi[q][b][lk] : (ph.convection ? ph.alpha * S.cphi[q][b] : 0.0));
+        Sacc = mfma16(A, Bv, Sacc);
+      }
+#pragma unroll 1
+      for (int cd = 0; cd < 9; ++cd) {
+        const int c = cd / 3, d = cd % 3;
+        d4_t acc = c == d ? Sacc : zero;
+        if (ph.convection == 2) {
+#pragma unroll 1
+          for (int st = 0; st < 7; ++st) {
+            const int q = 4 * st + lk;
+            const double A = (av && q < NQ) ? S.q2[q][a] : 0.0;
+            const double Bv = (bv && q < NQ) ? S.wN[q][cd] * S.q2[q][b] : 0.0;
+            acc = mfma16(A, Bv, acc);
+          }
+        }
+        if (ph.zeta_u != 0.0) {
+          const double A = av ? S.D[lk][c * 27 + a] : 0.0;
+          const double Bv = bv ? ph.zeta_u * S.MD[lk][d * 27 + b] : 0.0;
+          acc = mfma16(A, Bv, acc);
+        }
+        const int r0 = c * 27 + 16 * ti, c0 = d * 27 + 16 * tj;
+        scatter_tile(S, acc, 1.0, r0, min(16, 27 - 16 * ti), c0, min(16, 27 - 16 * tj), rk, val,
+                     [&](int i, int j) { return (r0 + i) * ROWS_U_LEN + (c0 + j); });
+      }
+    } else if (job < 13) {
+      const int t = job - 4, ti = t / 3, tj = t % 3;
+      const int r = 16 * ti + li, sidx = 16 * tj + li;
+      const bool rv = r < 36, sv = sidx < 36;
+      d4_t acc = zero;
+#pragma unroll 3
+      for (int q = 0; q < NQ; ++q) {
+        double A = 0.0, Bv = 0.0;
+        if (rv) A = lk < 3 ? S.psi[q][r][lk] : S.dpsi[q][r];
+        if (sv) Bv = S.W[q] * (lk < 3 ? S.psi[q][sidx][lk] : ph.zeta_j * S.dpsi[q][sidx]);
+        acc = mfma16(A, Bv, acc);
+      }
+      const int r0 = 16 * ti, s0 = 16 * tj;
+      scatter_tile(S, acc, 1.0, OFF_J + r0, min(16, 36 - r0), OFF_J + s0, min(16, 36 - s0), rk, val,
+                   [&](int i, int j) { return E_J + (r0 + i) * ROWS_J_LEN + NU + (s0 + j); });
+    } else {
+      const int t = job - 13, c = t / 6, tt = t % 6, ti = tt / 3, tj = tt % 3;
+      const int c1 = c == 2 ? 0 : c + 1, c2 = c == 0 ? 2 : c - 1;
+      const double b1 = S.B[c1], b2 = S.B[c2];
+      const int a = 16 * ti + li, r = 16 * tj + li;
+      const bool av = a < 27, rv = r < 36;
+      d4_t acc = zero;
+#pragma unroll 1
+      for (int st = 0; st < 7; ++st) {
+        const int q = 4 * st + lk;
+        const double A = (av && q < NQ) ? S.q2[q][a] : 0.0;
+        const double Bv = (rv && q < NQ) ? S.W[q] * (S.psi[q][r][c1] * b2 - S.psi[q][r][c2] * b1) : 0.0;
+        acc = mfma16(A, Bv, acc);
+      }
+      const int a0 = 16 * ti, r0 = 16 * tj;
+      const int na = min(16, 27 - a0), nr = min(16, 36 - r0);
+      // uj = -gamma X   (u row c*27 + a, j column r)
+      scatter_tile(S, acc, -ph.gamma, c * 27 + a0, na, OFF_J + r0, nr, rk, val,
+                   [&](int i, int j) { return (c * 27 + a0 + i) * ROWS_U_LEN + OFF_J + r0 + j; });
+      // ju = +sigma X^T (j row r, u column c*27 + a): lane holds X[a0+i][r0+j] -> entry (j row r0+j, u col a0+i)
+      {
+        const int j = lane & 15;
+        if (j < nr && S.g[OFF_J + r0 + j] >= 0) {
+          const int lr = OFF_J + r0 + j;
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int i = (lane >> 4) + 4 * rr;
+            if (i >= na) continue;
+            const int lc = c * 27 + a0 + i;
+            if (S.g[lc] < 0) continue;
+            atomicAdd(&val[S.rp[lr] + rk[E_J + (r0 + j) * ROWS_J_LEN + lc]], ph.sigma * acc[rr]);
+          }
+        }
+      }
+    }
+  }
+  // up / pu (81 x 4) and jphi / phij (36 x 8) on the vector ALUs
+  for (int t = threadIdx.x; t < NU * 4 + NJ * 8; t += WG) {
+    if (t < NU * 4) {
+      const int row = t >> 2, k = t & 3, ca = row / 27, na = row % 27;
+      if (S.g[row] < 0 || S.g[OFF_P + k] < 0) continue;
+      double acc = 0.0;
+      for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.p1[q][k] * S.gphi[q][na][ca];
+      atomicAdd(&val[S.rp[row] + rk[row * ROWS_U_LEN + OFF_P + k]], acc);                 // - dp div v
+      atomicAdd(&val[S.rp[OFF_P + k] + rk[E_P + k * ROWS_P_LEN + row]], acc);             // - div du q
+    } else {
+      const int tt = t - NU * 4, r = tt >> 3, v = tt & 7;
+      if (S.g[OFF_J + r] < 0 || S.g[OFF_PHI + v] < 0) continue;
+      double acc = 0.0;
+      for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.q1[q][v] * S.dpsi[q][r];
+      atomicAdd(&val[S.rp[OFF_J + r] + rk[E_J + r * ROWS_J_LEN + NU + NJ + v]], ph.sigma * acc);  // - sigma dphi div s
+      atomicAdd(&val[S.rp[OFF_PHI + v] + rk[E_PHI + v * ROWS_PHI_LEN + r]], acc);                 // - div dj phi_t
+    }
+  }
+}
+
+__global__ __launch_bounds__(WG, 2) void k_cell_jacobian(int ncells, const double *__restrict__ xyz,
                                                       const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
                                                       const double *__restrict__ dir, const double *__restrict__ x,
                                                       const RefTables *__restrict__ T, const int64_t *__restrict__ rowptr,
@@ -376,19 +554,14 @@ __global__ __launch_bounds__(WG) void k_cell_jacobian(int ncells, const double *
   __shared__ CellLDS S;
   for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
     stage_cell(S, c, xyz, dofs, sign, dir, x, T, rowptr, ph, ph.zeta_u != 0.0);
-    const uint16_t *rk = rank + (int64_t)c * NENT;
-    for (int e = threadIdx.x; e < NENT; e += WG) {
-      int a, b;
-      decode_entry(e, a, b);
-      if (S.g[a] < 0 || S.g[b] < 0) continue;
-      const double v = jac_entry(S, T, ph, a, b);
-      atomicAdd(&val[S.rp[a] + rk[e]], v);
-    }
+    if (threadIdx.x < 3) S.B[threadIdx.x] = ph.B[threadIdx.x];
+    __syncthreads();
+    cell_jacobian_tiles(S, ph, rank + (int64_t)c * NENT, val);
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(WG) void k_cell_residual(int ncells, const double *__restrict__ xyz,
+__global__ __launch_bounds__(WG, 2) void k_cell_residual(int ncells, const double *__restrict__ xyz,
                                                       const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
                                                       const double *__restrict__ dir, const double *__restrict__ x,
                                                       const RefTables *__restrict__ T, double *__restrict__ r,
@@ -398,6 +571,40 @@ __global__ __launch_bounds__(WG) void k_cell_residual(int ncells, const double *
     stage_cell(S, c, xyz, dofs, sign, dir, x, T, nullptr, ph, ph.zeta_u != 0.0);
     const int a = threadIdx.x;
     if (a < NDOF && S.g[a] >= 0) atomicAdd(&r[S.g[a]], res_entry(S, T, ph, a));
+    __syncthreads();
+  }
+}
+
+// Per-cell integrals of the state (post-processing, src/Applications/hunt.jl:247-260):
+// out[c] = (int u.u, int grad u : grad u, int j.j, int (div j)^2, int p, |K|).  The 3x3x3
+// Gauss rule integrates these polynomial quantities exactly on affine cells.
+__global__ __launch_bounds__(WG, 2) void k_cell_integrals(int ncells, const double *__restrict__ xyz,
+                                                         const int32_t *__restrict__ dofs,
+                                                         const int8_t *__restrict__ sign,
+                                                         const double *__restrict__ dir,
+                                                         const double *__restrict__ x,
+                                                         const RefTables *__restrict__ T, double *__restrict__ out,
+                                                         MhdPhys ph) {
+  __shared__ CellLDS S;
+  for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
+    stage_cell(S, c, xyz, dofs, sign, dir, x, T, nullptr, ph, false);
+    const int k = threadIdx.x;
+    if (k < 6) {
+      double acc = 0.0;
+      for (int q = 0; q < NQ; ++q) {
+        double v;
+        if (k == 0) v = S.u[q][0] * S.u[q][0] + S.u[q][1] * S.u[q][1] + S.u[q][2] * S.u[q][2];
+        else if (k == 1) {
+          v = 0.0;
+          for (int i = 0; i < 9; ++i) v += S.gu[q][i] * S.gu[q][i];
+        } else if (k == 2) v = S.jv[q][0] * S.jv[q][0] + S.jv[q][1] * S.jv[q][1] + S.jv[q][2] * S.jv[q][2];
+        else if (k == 3) v = S.divj[q] * S.divj[q];
+        else if (k == 4) v = S.p[q];
+        else v = 1.0;
+        acc += S.W[q] * v;
+      }
+      out[(int64_t)c * 6 + k] = acc;
+    }
     __syncthreads();
   }
 }
@@ -662,6 +869,13 @@ int mhd_launch_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t 
                        c->d_dir, d_x, c->d_fref, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw, d_r);
     MHD_HIP(hipGetLastError());
   }
+  return 0;
+}
+
+int mhd_launch_cell_integrals(mhd_ctx *c, const double *d_x, double *d_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_cell_integrals, dim3(c->ncells), dim3(WG), 0, s, c->ncells, c->d_xyz, c->d_dofs, c->d_sign,
+                     c->d_dir, d_x, c->d_ref, d_out, c->phys);
+  MHD_HIP(hipGetLastError());
   return 0;
 }
 

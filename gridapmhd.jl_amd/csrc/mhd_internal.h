@@ -108,6 +108,7 @@ int mhd_launch_jacobian(mhd_ctx *c, const double *d_x, double *d_val, hipStream_
 int mhd_build_rank_map(mhd_ctx *c);
 int mhd_build_tperm(mhd_ctx *c);
 int mhd_launch_spmv(mhd_ctx *c, const double *d_val, const double *d_x, double *d_y, hipStream_t s);
+int mhd_launch_cell_integrals(mhd_ctx *c, const double *d_x, double *d_out, hipStream_t s);
 
 #define MHD_HIP(call)                                                                 \
   do {                                                                                \

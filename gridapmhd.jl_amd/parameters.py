@@ -1,0 +1,199 @@
+"""Parameter schema of ``GridapMHD.main`` restricted to the hot path.
+
+Mirrors src/parameters.jl: ``add_default_params`` (:85-137), the
+mandatory/optional/unused-key checks (:3-76), ``params_fluid`` (:693-723),
+``params_bcs`` and its sub-dicts (:777-947), ``params_fespaces`` (:350-393)
+and ``default_solver_params`` (:219-287).  Keys keep the reference's names
+(``"α"``, ``"ζᵤ"``, ``"thin_wall"``, ``"τ"``...).  Inputs outside the
+hexahedral H1-HDiv path (other formulations, solids, transient,
+continuation, multigrid) raise ``NotImplementedError``.
+"""
+from __future__ import annotations
+
+import copy
+from typing import Any, Dict
+
+import numpy as np
+
+
+class ParamsError(ValueError):
+    pass
+
+
+def _check_mandatory(params, mandatory, p):
+    if not isinstance(params, dict):
+        raise ParamsError(f"The params{p} has to be a dict")
+    for key, req in mandatory.items():
+        if req and key not in params:
+            raise ParamsError(f"Key :{key} is a mandatory key in params{p}, but it is not provided.")
+
+
+def _add_optional(sub, mandatory, optional, p):
+    out = dict(sub)
+    for key, req in mandatory.items():
+        if not req and key not in optional:
+            raise ParamsError("Internal error")
+    for key, val in optional.items():
+        out.setdefault(key, val)
+    return out
+
+
+def _check_unused(sub, mandatory, params, p):
+    if params.get("check_valid", True):
+        for key in sub:
+            if key not in mandatory:
+                raise ParamsError(f"Key :{key} is not a valid key in params{p}. "
+                                  f"Set params[:check_valid] = false to ignore invalid keys.")
+
+
+def _check_and_add(sub, mandatory, optional, params, p):
+    _check_mandatory(sub, mandatory, p)
+    out = _add_optional(sub, mandatory, optional, p)
+    _check_unused(out, mandatory, params, p)
+    return out
+
+
+def _check_and_add_weak(t, mandatory, optional, params, p):
+    if isinstance(t, dict):
+        return [_check_and_add(t, mandatory, optional, params, p)]
+    if isinstance(t, (list, tuple)):
+        return [_check_and_add(ti, mandatory, optional, params, f"{p}[{i}]") for i, ti in enumerate(t)]
+    raise ParamsError(f"The value params{p} has to be a dict or a list of dicts.")
+
+
+# ---------------------------------------------------------------------------
+def default_solver_params(name: str) -> Dict[str, Any]:
+    """``default_solver_params(::Val{S})``.  ``"hip"`` is this backend; the
+    Badia2024 name maps onto it (same preconditioner, on the GPU)."""
+    if name in ("hip", "badia2024"):
+        return {
+            "solver": name,
+            "matrix_type": "SparseMatrixCSC{Float64,Int64}",
+            "vector_type": "Vector{Float64}",
+            "solver_postpro": hip_postpro,
+            "niter": 20,          # Newton iterations
+            "niter_ls": 400,      # FGMRES iterations per Newton step
+            "rtol": 1e-6,
+            "atol": 1e-8,
+            "precision": "fp64",  # "fp64" | "mixed" | "mixed_bf16"
+            "restart": 100,
+            "inner_iters": 8,
+            "block_solvers": ["hip_vertex_schwarz", "mass_inverse", "mass_inverse"],
+        }
+    raise NotImplementedError(f"solver {name!r}: this backend provides 'hip' (and 'badia2024')")
+
+
+def hip_postpro(info, output):
+    """``BlockSolver_postpro`` (src/parameters.jl:323-338) for the GPU solver."""
+    output["nls_iters"] = int(info.newton_iters)
+    output["ls_iters"] = int(info.krylov_iters)
+    output["nls_residuals"] = [float(info.newton_residuals[k]) for k in range(min(64, info.newton_iters + 1))]
+    output["ls_iters_per_step"] = [int(info.krylov_per_step[k]) for k in range(min(64, info.newton_iters))]
+
+
+def params_solver(params):
+    s = params["solver"]
+    if isinstance(s, str):
+        return default_solver_params(s)
+    if "solver" not in s:
+        raise ParamsError("params[:solver] needs the key :solver")
+    out = dict(default_solver_params(s["solver"]))
+    out.update(s)
+    return out
+
+
+def params_fespaces(params):
+    fe = params.get("fespaces") or {}
+    mandatory = {"order_u": False, "order_j": False, "p_constraint": False, "φ_constraint": False,
+                 "rt_scaling": False, "fluid_disc": False, "current_disc": False}
+    order_u = fe.get("order_u", 2)
+    optional = {"order_u": 2, "order_j": order_u, "p_constraint": None, "φ_constraint": None,
+                "rt_scaling": None, "fluid_disc": "Qk_dPkm1", "current_disc": "RT"}
+    fe = _check_and_add(fe, mandatory, optional, params, "[:fespaces]")
+    if fe["order_u"] != 2 or fe["order_j"] != 2:
+        raise NotImplementedError("the GPU kernels implement order_u = order_j = 2 (Q2 / RT1)")
+    if fe["fluid_disc"] != "Qk_dPkm1" or fe["current_disc"] != "RT":
+        raise NotImplementedError("the GPU kernels implement the :Qk_dPkm1 / :RT hexahedral pair (H1HDiv)")
+    if fe["p_constraint"] is not None or fe["φ_constraint"] is not None or fe["rt_scaling"] is not None:
+        raise NotImplementedError("p / φ constraints and rt_scaling are not supported")
+    fe["formulation"] = "H1HDiv"
+    fe["order_p"] = fe["order_φ"] = 1
+    fe["k"] = 2
+    # quadrature degree q = max(2(k-1), 3k-1, 2k, k+k, 2(k-1)) = 5  (src/parameters.jl:382-388)
+    fe["q"] = max(2 * (2 - 1), 3 * 2 - 1, 2 * 2, 2 + 2, 2 * (1 - 1))
+    return fe
+
+
+def params_fluid(params):
+    mandatory = {"domain": True, "α": True, "β": True, "γ": True, "B": True, "f": False, "σ": False, "ζᵤ": False,
+                 "ζⱼ": False, "g": False, "divg": False, "convection": False, "μ": False}
+    optional = {"σ": 1.0, "f": (0.0, 0.0, 0.0), "ζᵤ": 0.0, "ζⱼ": 0.0, "g": (0.0, 0.0, 0.0), "divg": 0.0,
+                "convection": "newton", "μ": 2 * 3}
+    fluid = _check_and_add(params["fluid"], mandatory, optional, params, "[:fluid]")
+    if fluid["convection"] not in ("none", "newton", "picard"):
+        raise ParamsError("fluid[:convection] must be one of none, newton, picard")
+    if fluid["domain"] is not None:
+        raise NotImplementedError("fluid sub-domains (solid regions) are outside the hot path")
+    return fluid
+
+
+def params_bcs(params):
+    mandatory = {"u": True, "j": True, "φ": False, "t": False, "thin_wall": False, "stabilization": False}
+    optional = {"j": [], "φ": [], "t": [], "thin_wall": [], "stabilization": []}
+    bcs = _check_and_add(params["bcs"], mandatory, optional, params, "[:bcs]")
+    bcs["u"] = _check_and_add(bcs["u"], {"tags": True, "values": False}, {"values": None}, params, "[:bcs][:u]")
+    if bcs["j"] != []:
+        bcs["j"] = _check_and_add(bcs["j"], {"tags": True, "values": False}, {"values": None}, params, "[:bcs][:j]")
+    else:
+        bcs["j"] = {"tags": [], "values": None}
+    if bcs["φ"] != []:
+        raise NotImplementedError("weak potential BCs (res_φ_bcs) are not part of the GPU path")
+    if bcs["t"] != []:
+        raise NotImplementedError("Boundary tranction not yet implemented")  # as src/weakforms.jl:164-166
+    if bcs["stabilization"] != []:
+        raise NotImplementedError("skeleton stabilisation is not part of the GPU path")
+    if bcs["thin_wall"] != []:
+        bcs["thin_wall"] = _check_and_add_weak(bcs["thin_wall"], {"domain": True, "cw": True, "τ": True, "jw": False},
+                                               {"jw": 0.0}, params, "[:bcs][:thin_wall]")
+    return bcs
+
+
+def add_default_params(_params: Dict[str, Any]) -> Dict[str, Any]:
+    """``add_default_params`` (src/parameters.jl:85-137)."""
+    mandatory = {"ptimer": False, "debug": False, "solve": False, "res_assemble": False, "jac_assemble": False,
+                 "model": True, "fluid": True, "solid": False, "bcs": True, "fespaces": False, "solver": True,
+                 "multigrid": False, "check_valid": False, "transient": False, "continuation": False, "x0": False}
+    _check_mandatory(_params, mandatory, "")
+    optional = {"ptimer": None, "debug": False, "solve": True, "res_assemble": False, "jac_assemble": False,
+                "solid": None, "fespaces": None, "multigrid": None, "check_valid": True, "transient": None,
+                "continuation": None, "x0": "zero"}
+    params = _add_optional(_params, mandatory, optional, "")
+    _check_unused(params, mandatory, params, "")
+    for k in ("solid", "multigrid", "transient", "continuation"):
+        if params[k] is not None:
+            raise NotImplementedError(f"params[:{k}] is outside the stationary H1-HDiv hot path")
+    params = dict(params)
+    params["fespaces"] = params_fespaces(params)
+    params["fluid"] = params_fluid(params)
+    params["bcs"] = params_bcs(params)
+    params["solver"] = params_solver(params)
+    return params
+
+
+def duplicate_params(params):
+    """``duplicate_params`` (src/parameters.jl:148-158)."""
+    return {k: duplicate_params(v) if isinstance(v, dict) else v for k, v in params.items()}
+
+
+def fluid_physics(fluid) -> Dict[str, Any]:
+    """The fluid dict as the flat parameter set of the C ABI."""
+    def vec_or_affine(v):
+        # constant vector, or (v0, M) meaning v0 + M x
+        if isinstance(v, tuple) and len(v) == 2 and np.ndim(v[1]) == 2:
+            return np.asarray(v[0], float), np.asarray(v[1], float)
+        return np.asarray(v, float), np.zeros((3, 3))
+    f0, fx = vec_or_affine(fluid["f"])
+    g0, gx = vec_or_affine(fluid["g"])
+    return dict(alpha=float(fluid["α"]), beta=float(fluid["β"]), gamma=float(fluid["γ"]), sigma=float(fluid["σ"]),
+                B=np.asarray(fluid["B"], float), f0=f0, fx=fx, g0=g0, gx=gx, zeta_u=float(fluid["ζᵤ"]),
+                zeta_j=float(fluid["ζⱼ"]), convection=fluid["convection"])

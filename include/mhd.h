@@ -129,6 +129,11 @@ int mhd_residual(mhd_ctx *ctx, const double *x_free, double *r_free);
 /* jacobian(op, xh) (src/weakforms.jl:234-253 + 728-733): nzval in CSC order of mhd_pattern. */
 int mhd_jacobian(mhd_ctx *ctx, const double *x_free, double *nzval);
 
+/* Per-cell integrals of a state, out[6 * ncells]: (int u.u, int grad u : grad u,
+ * int j.j, int (div j)^2, int p, |K|) -- the post-processing norms of _hunt
+ * (src/Applications/hunt.jl:247-260) summed by the caller. */
+int mhd_cell_integrals(mhd_ctx *ctx, const double *x_free, double *out);
+
 /* Device-pointer variants (inputs resident in HBM).  The Jacobian values are
  * written in the library's CSR order (row-major, columns sorted); the pattern
  * is structurally symmetric, so CSR(A) and CSC(A) share index arrays and the
