@@ -70,6 +70,11 @@ EXPORTS = {
                                         ctypes.POINTER(MhdSolveInfo)]),
     "mhd_default_solver_opts": (None, [ctypes.POINTER(MhdSolverOpts)]),
     "mhd_precond_setup": (ctypes.c_int, [c_void_p, ctypes.POINTER(MhdSolverOpts)]),
+    "mhd_dresidual_dparam": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_void_p]),
+    "mhd_adjoint_solve": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(MhdSolverOpts),
+                                         ctypes.POINTER(MhdSolveInfo)]),
+    "mhd_attach_coarse": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                         c_int32, c_double]),
     "mhd_smoother_apply_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhd_smoother_info": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
                                          ctypes.POINTER(c_int64)]),
@@ -228,6 +233,36 @@ class Context:
             setattr(o, k, v)
         return o
 
+    def attach_coarse(self, coarse: "Context", P, inj_u, coarse_iters: int = 10, smoother_weight: float = 0.2):
+        """Hand `coarse` (and the prolongation P, scipy CSR) to this context."""
+        rowptr = np.ascontiguousarray(P.indptr, dtype=np.int64)
+        col = np.ascontiguousarray(P.indices, dtype=np.int32)
+        val = np.ascontiguousarray(P.data, dtype=np.float64)
+        inj = np.ascontiguousarray(inj_u, dtype=np.int32)
+        self._check(self.lib.mhd_attach_coarse(self.h, coarse.h, P.shape[0], P.shape[1], _ptr(rowptr), _ptr(col),
+                                               _ptr(val), _ptr(inj), int(coarse_iters), float(smoother_weight)),
+                    "mhd_attach_coarse")
+        coarse.h = c_void_p()          # owned by this context now
+        self._coarse = coarse
+
+    def dresidual_dparam(self, x, param: int):
+        """dR/dtheta at x: param 0 = gamma (Lorentz term), 1 = cw (thin walls)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty(self.n)
+        self._check(self.lib.mhd_dresidual_dparam(self.h, _ptr(x), int(param), _ptr(out)), "mhd_dresidual_dparam")
+        return out
+
+    def adjoint_solve(self, x, rhs, opts: Optional[MhdSolverOpts] = None):
+        """lambda = A(x)^-T rhs on the GPU; returns (lambda, info, rc)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        lam = np.empty(self.n)
+        opts = opts or self.default_opts()
+        info = MhdSolveInfo()
+        rc = self._check(self.lib.mhd_adjoint_solve(self.h, _ptr(x), _ptr(rhs), _ptr(lam), ctypes.byref(opts),
+                                                    ctypes.byref(info)), "mhd_adjoint_solve")
+        return lam, info, rc
+
     def precond_setup(self, opts: MhdSolverOpts):
         return self._check(self.lib.mhd_precond_setup(self.h, ctypes.byref(opts)), "mhd_precond_setup")
 
@@ -249,6 +284,27 @@ class Context:
         rc = self._check(self.lib.mhd_newton_solve(self.h, _ptr(x), ctypes.byref(opts), ctypes.byref(info)),
                          "mhd_newton_solve")
         return x, info, rc
+
+
+def context_with_levels(sp, phys, levels=1, coarse_iters=10, smoother_weight=0.2, device=0) -> Context:
+    """Fine context plus `levels - 1` semi-coarsened levels attached for the
+    [u,j] V-cycle (badia2024 numbering, nested x/y coarsening, the same
+    boundary conditions on every level)."""
+    from . import fespaces, multigrid
+    ctx = context_from_spaces(sp, phys, device=device)
+    if levels <= 1:
+        return ctx
+    bc = sp.bc_spec
+    chain = [sp]
+    for _ in range(levels - 1):
+        mesh = multigrid.coarsen_mesh(chain[-1].mesh)
+        chain.append(fespaces.setup_fe_spaces(mesh, bc["u_tags"], bc["j_tags"], bc["thin_walls"], style=sp.style))
+    ctxs = [ctx] + [context_from_spaces(c, phys, device=device) for c in chain[1:]]
+    # attach from the coarsest upwards so each attached context already carries its chain
+    for lev in range(levels - 1, 0, -1):
+        P, inj = multigrid.prolongation(chain[lev - 1], chain[lev])
+        ctxs[lev - 1].attach_coarse(ctxs[lev], P, inj, coarse_iters, smoother_weight)
+    return ctx
 
 
 def context_from_spaces(sp, phys, dir_vals=None, device=0) -> Context:

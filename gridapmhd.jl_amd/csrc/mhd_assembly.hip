@@ -339,8 +339,18 @@ __device__ double jac_entry(const CellLDS &S, const RefTables *__restrict__ T, c
 }
 
 // Residual entry of test function a.
-__device__ double res_entry(const CellLDS &S, const RefTables *__restrict__ T, const MhdPhys &ph, int a) {
+// mode 0: residual; mode 1: dR/dgamma = -int (j x B).v (u rows only)
+__device__ double res_entry(const CellLDS &S, const RefTables *__restrict__ T, const MhdPhys &ph, int a, int mode) {
   double acc = 0.0;
+  if (mode == 1) {
+    if (a >= OFF_P) return 0.0;
+    const int ca = a / 27, na = a % 27;
+    const int i1 = ca == 2 ? 0 : ca + 1, i2 = ca == 0 ? 2 : ca - 1;
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q)
+      acc -= S.W[q] * (S.jv[q][i1] * S.B[i2] - S.jv[q][i2] * S.B[i1]) * S.q2[q][na];
+    return acc;
+  }
   if (a < OFF_P) {
     const int ca = a / 27, na = a % 27;
 #pragma unroll 1
@@ -565,12 +575,12 @@ __global__ __launch_bounds__(WG, 2) void k_cell_residual(int ncells, const doubl
                                                       const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
                                                       const double *__restrict__ dir, const double *__restrict__ x,
                                                       const RefTables *__restrict__ T, double *__restrict__ r,
-                                                      MhdPhys ph) {
+                                                      MhdPhys ph, int mode) {
   __shared__ CellLDS S;
   for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
     stage_cell(S, c, xyz, dofs, sign, dir, x, T, nullptr, ph, ph.zeta_u != 0.0);
     const int a = threadIdx.x;
-    if (a < NDOF && S.g[a] >= 0) atomicAdd(&r[S.g[a]], res_entry(S, T, ph, a));
+    if (a < NDOF && S.g[a] >= 0) atomicAdd(&r[S.g[a]], res_entry(S, T, ph, a, mode));
     __syncthreads();
   }
 }
@@ -722,7 +732,7 @@ __global__ __launch_bounds__(64) void k_face_residual(int nfaces, const double *
                                                       const FaceTables *__restrict__ F, const int32_t *__restrict__ fcell,
                                                       const int8_t *__restrict__ flf, const double *__restrict__ ftau,
                                                       const double *__restrict__ fcw, const double *__restrict__ fjw,
-                                                      double *__restrict__ r) {
+                                                      double *__restrict__ r, int mode) {
   __shared__ FaceLDS S;
   for (int f = blockIdx.x; f < nfaces; f += gridDim.x) {
     int cell, lf;
@@ -734,7 +744,7 @@ __global__ __launch_bounds__(64) void k_face_residual(int nfaces, const double *
       for (int q = 0; q < NQF; ++q) {
         const double *s = S.psi[q][a], *n = S.n[q];
         const double sn = s[0] * n[0] + s[1] * n[1] + s[2] * n[2];
-        acc += S.W[q] * tau * sn * (S.jn[q] - jw + cw * S.jnn[q]);
+        acc += mode == 2 ? S.W[q] * tau * sn * S.jnn[q] : S.W[q] * tau * sn * (S.jn[q] - jw + cw * S.jnn[q]);
       }
       atomicAdd(&r[S.g[a]], acc);
     }
@@ -857,18 +867,41 @@ int mhd_launch_jacobian(mhd_ctx *c, const double *d_x, double *d_val, hipStream_
   return 0;
 }
 
-int mhd_launch_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s) {
+int mhd_launch_residual_mode(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s, int mode) {
   MHD_HIP(hipMemsetAsync(d_r, 0, sizeof(double) * c->n, s));
-  if (c->ncells > 0) {
+  if (c->ncells > 0 && mode != 2) {
     hipLaunchKernelGGL(k_cell_residual, dim3(c->ncells), dim3(WG), 0, s, c->ncells, c->d_xyz, c->d_dofs, c->d_sign,
-                       c->d_dir, d_x, c->d_ref, d_r, c->phys);
+                       c->d_dir, d_x, c->d_ref, d_r, c->phys, mode);
     MHD_HIP(hipGetLastError());
   }
-  if (c->nfaces > 0) {
+  if (c->nfaces > 0 && mode != 1) {
     hipLaunchKernelGGL(k_face_residual, dim3(c->nfaces), dim3(64), 0, s, c->nfaces, c->d_xyz, c->d_dofs, c->d_sign,
-                       c->d_dir, d_x, c->d_fref, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw, d_r);
+                       c->d_dir, d_x, c->d_fref, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw, d_r, mode);
     MHD_HIP(hipGetLastError());
   }
+  return 0;
+}
+
+int mhd_launch_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s) {
+  return mhd_launch_residual_mode(c, d_x, d_r, s, 0);
+}
+
+// val_t[p] = val[tperm[p]]: the CSR values of A^T on the (symmetric) pattern
+__global__ void k_gather_t(int64_t nnz, const int64_t *__restrict__ tperm, const double *__restrict__ val,
+                           double *__restrict__ val_t) {
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x)
+    val_t[p] = val[tperm[p]];
+}
+
+int mhd_launch_transpose_values(mhd_ctx *c, const double *d_val, double *d_val_t, hipStream_t s) {
+  int rc;
+  if (!c->d_tperm) {
+    MHD_HIP(hipMalloc(&c->d_tperm, sizeof(int64_t) * std::max<int64_t>(c->nnz, 1)));
+    if ((rc = mhd_build_tperm(c))) return rc;
+  }
+  const int grid = (int)std::min<int64_t>((c->nnz + 255) / 256, 1 << 20);
+  hipLaunchKernelGGL(k_gather_t, dim3(grid), dim3(256), 0, s, c->nnz, c->d_tperm, d_val, d_val_t);
+  MHD_HIP(hipGetLastError());
   return 0;
 }
 

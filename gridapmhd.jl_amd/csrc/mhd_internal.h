@@ -68,6 +68,7 @@ struct MhdPhys {
 struct mhd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;
   std::string err;
   int32_t ncells = 0, nfaces = 0, ndir = 0;
   int64_t n = 0, nnz = 0;
@@ -100,7 +101,29 @@ struct mhd_ctx {
   int32_t block_style = 0;
   // solver workspace (allocated lazily)
   struct Solver *solver = nullptr;
+  // coarser level of the [u,j] block solve (owned), see mhd_attach_coarse
+  struct CoarseLink *coarse = nullptr;
 };
+
+// Link from a level to the next coarser one: prolongation P (n1 x n1c CSR),
+// its transpose, the u-state injection and V-cycle temporaries.
+struct CoarseLink {
+  mhd_ctx *ctx = nullptr;          // coarse context (owned)
+  int64_t n1 = 0, n1c = 0, nuc = 0;
+  int64_t *d_Prow = nullptr, *d_Trow = nullptr;
+  int32_t *d_Pcol = nullptr, *d_Tcol = nullptr;
+  double *d_Pval = nullptr, *d_Tval = nullptr;
+  float *d_Pval32 = nullptr, *d_Tval32 = nullptr;
+  int32_t *d_inj = nullptr;        // [nuc] fine u dof of every coarse u dof
+  int coarse_iters = 10;           // Schwarz-GMRES steps on the coarsest level
+  double omega = 0.2;              // damping of the Schwarz smoother inside the V-cycle
+  void *rr = nullptr;              // [n1] fine residual (T)
+  void *rc = nullptr;              // [n1c] coarse rhs (T)
+  void *ec = nullptr;              // [n1c] coarse correction (T)
+  int tprec = -1;                  // element type of rr/rc/ec: 0 double, 1 float
+};
+void mhd_coarse_free(mhd_ctx *c);
+void mhd_solver_free(mhd_ctx *c);
 
 // Launchers (mhd_assembly.hip)
 int mhd_launch_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s);
@@ -109,6 +132,8 @@ int mhd_build_rank_map(mhd_ctx *c);
 int mhd_build_tperm(mhd_ctx *c);
 int mhd_launch_spmv(mhd_ctx *c, const double *d_val, const double *d_x, double *d_y, hipStream_t s);
 int mhd_launch_cell_integrals(mhd_ctx *c, const double *d_x, double *d_out, hipStream_t s);
+int mhd_launch_residual_mode(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s, int mode);
+int mhd_launch_transpose_values(mhd_ctx *c, const double *d_val, double *d_val_t, hipStream_t s);
 
 #define MHD_HIP(call)                                                                 \
   do {                                                                                \

@@ -131,6 +131,7 @@ __global__ __launch_bounds__(TB) void k_lincomb(int K, const T *__restrict__ Z, 
 
 // ---------------------------------------------------------------- SpMV
 // MODE 0: full rows.  MODE 1: A11 prefix [rowptr, split).  MODE 2: y = v - suffix [split, rowptr+1) . x
+// MODE 3: y = v - prefix . x
 template <class MV, class TX, class TY, int LPR, int MODE>
 __global__ __launch_bounds__(256) void k_spmv_seg(int64_t nrows, const int64_t *__restrict__ rowptr,
                                                   const int64_t *__restrict__ split, const int32_t *__restrict__ colind,
@@ -142,16 +143,39 @@ __global__ __launch_bounds__(256) void k_spmv_seg(int64_t nrows, const int64_t *
   double acc = 0.0;
   if (row < nrows) {
     int64_t b = rowptr[row], e = rowptr[row + 1];
-    if (MODE == 1) e = split[row];
+    if (MODE == 1 || MODE == 3) e = split[row];
     if (MODE == 2) b = split[row];
     for (int64_t p = b + lane; p < e; p += LPR) acc += (double)val[p] * (double)x[colind[p]];
   }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
   if (row < nrows && lane == 0) {
-    if (MODE == 2) y[row] = (TY)((double)v[row] - acc);
+    if (MODE == 2 || MODE == 3) y[row] = (TY)((double)v[row] - acc);
     else y[row] = (TY)acc;
   }
+}
+
+// y (+)= A x for a plain CSR matrix (prolongation / restriction)
+template <class MV, class T, int LPR, bool ACC>
+__global__ __launch_bounds__(256) void k_csr(int64_t nrows, const int64_t *__restrict__ rowptr,
+                                             const int32_t *__restrict__ colind, const MV *__restrict__ val,
+                                             const T *__restrict__ x, T *__restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gid / LPR;
+  const int lane = threadIdx.x % LPR;
+  double acc = 0.0;
+  if (row < nrows)
+    for (int64_t p = rowptr[row] + lane; p < rowptr[row + 1]; p += LPR) acc += (double)val[p] * (double)x[colind[p]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
+  if (row < nrows && lane == 0) y[row] = ACC ? (T)((double)y[row] + acc) : (T)acc;
+}
+
+// coarse state: u by injection from the fine state, everything else zero
+__global__ void k_inject(int64_t nc, int64_t nuc, const int32_t *__restrict__ inj, const double *__restrict__ xf,
+                         double *__restrict__ xc) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc; i += (int64_t)gridDim.x * blockDim.x)
+    xc[i] = i < nuc ? xf[inj[i]] : 0.0;
 }
 
 __global__ void k_split(int64_t n1, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
@@ -223,11 +247,11 @@ __global__ void k_patch_extract(int npatch, int PS, const int32_t *__restrict__ 
   }
 }
 
-// y[P] += A_P^-1 r[P]  (one workgroup per patch, inverse column-major)
+// y[P] += w A_P^-1 r[P]  (one workgroup per patch, inverse column-major)
 template <class MP, class T>
 __global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const int32_t *__restrict__ psize,
                                                      const int32_t *__restrict__ pidx, const MP *__restrict__ Ainv,
-                                                     const T *__restrict__ r, T *__restrict__ y) {
+                                                     const T *__restrict__ r, T *__restrict__ y, double w) {
   __shared__ double rl[256];
   const int pch = blockIdx.x;
   const int sz = psize[pch];
@@ -239,7 +263,7 @@ __global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const i
     const MP *Ap = Ainv + (int64_t)pch * PS * PS + t;
     double acc = 0.0;
     for (int b = 0; b < sz; ++b) acc += todbl(Ap[(int64_t)b * PS]) * rl[b];
-    atomicAdd(&y[ids[t]], (T)acc);
+    atomicAdd(&y[ids[t]], (T)(w * acc));
   }
 }
 
@@ -268,6 +292,7 @@ struct Solver {
   double *V = nullptr, *Z = nullptr, *w = nullptr, *b = nullptr, *dx = nullptr, *t1 = nullptr;
   void *V1 = nullptr, *Z1 = nullptr, *w1 = nullptr, *x1 = nullptr, *r1 = nullptr;  // inner, n1-long, FP64 or FP32
   int inner_prec = -1;
+  bool transposed = false;  // preconditioning A^T (adjoint solve): lower block-triangular
   double *d_part = nullptr, *d_dots = nullptr, *d_y = nullptr;
   double *h_dots = nullptr;  // pinned
   int nb_dot = 0;
@@ -463,6 +488,17 @@ int solver_init(mhd_ctx *c, const mhd_solver_opts *o) {
   if ((rc = dmalloc(c, &S->d_y, 256))) return rc;
   MHD_HIP(hipHostMalloc(reinterpret_cast<void **>(&S->h_dots), sizeof(double) * 512, hipHostMallocDefault));
   MHD_HIP(hipStreamSynchronize(c->stream));
+  if (c->coarse) {
+    mhd_ctx *cc = c->coarse->ctx;
+    mhd_solver_opts oc = *o;
+    oc.restart = 2;  // coarse levels only run the inner solve
+    oc.inner_iters = cc->coarse ? 0 : c->coarse->coarse_iters;
+    if (cc->solver && (cc->solver->kin != std::max(0, std::min(oc.inner_iters, 120)))) mhd_solver_free(cc);
+    if ((rc = solver_init(cc, &oc))) {
+      c->err = "coarse level: " + cc->err;
+      return rc;
+    }
+  }
   return 0;
 }
 
@@ -565,19 +601,20 @@ void hess_solve(int k, const std::vector<double> &H, int ldh, double beta, std::
 }
 
 // Additive Schwarz over the vertex patches: y = sum_P R_P^T A_P^-1 R_P r
+// y (+)= w sum_P R_P^T A_P^-1 R_P r
 template <class T>
-int patch_as(mhd_ctx *c, const T *r, T *y, int prec) {
+int patch_as(mhd_ctx *c, const T *r, T *y, int prec, double w = 1.0, bool accumulate = false) {
   Solver *S = c->solver;
-  hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, (T)0, y);
+  if (!accumulate) hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, (T)0, y);
   if (prec == 0)
     hipLaunchKernelGGL((k_patch_apply<double, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch, S->PS,
-                       S->d_psize, S->d_pidx, S->d_pA, r, y);
+                       S->d_psize, S->d_pidx, S->d_pA, r, y, w);
   else if (prec == 1)
     hipLaunchKernelGGL((k_patch_apply<float, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch, S->PS,
-                       S->d_psize, S->d_pidx, S->d_pA32, r, y);
+                       S->d_psize, S->d_pidx, S->d_pA32, r, y, w);
   else
     hipLaunchKernelGGL((k_patch_apply<__hip_bfloat16, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch,
-                       S->PS, S->d_psize, S->d_pidx, S->d_pA16, r, y);
+                       S->PS, S->d_psize, S->d_pidx, S->d_pA16, r, y, w);
   MHD_HIP(hipGetLastError());
   return 0;
 }
@@ -597,14 +634,75 @@ int spmv_a11(mhd_ctx *c, const T *x, T *y, int prec) {
   return 0;
 }
 
-// x1 ~ A11^-1 r1 by kin steps of right-preconditioned GMRES from x = 0.
+template <class T>
+int inner_solve(mhd_ctx *c, int prec);
+
+// One V-cycle on the [u,j] block of level c: y = M^-1 r with vertex-star
+// Schwarz pre- and post-smoothing around the coarse correction
+// y += P A11c^-1 P^T (r - A11 y); the coarsest level is solved by
+// coarse_iters steps of Schwarz-preconditioned FGMRES.
+template <class T>
+int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
+  Solver *S = c->solver;
+  int rc;
+  CoarseLink *L = c->coarse;
+  if (!L) return patch_as<T>(c, r, y, prec);
+  if ((rc = patch_as<T>(c, r, y, prec, L->omega))) return rc;
+  mhd_ctx *cc = L->ctx;
+  Solver *SC = cc->solver;
+  constexpr int LPR = 16;
+  T *rr = (T *)L->rr, *rcv = (T *)L->rc, *ec = (T *)L->ec;
+  const int gfine = (int)((S->n1 * LPR + 255) / 256), gc = (int)((L->n1c * LPR + 255) / 256);
+  auto resid = [&]() {
+    if (prec == 0)
+      hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1,
+                         c->d_rowptr, S->d_split, c->d_colind, c->d_val, y, r, rr);
+    else
+      hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1,
+                         c->d_rowptr, S->d_split, c->d_colind, S->d_val32, y, r, rr);
+  };
+  resid();
+  if (prec == 0)
+    hipLaunchKernelGGL((k_csr<double, T, LPR, false>), dim3(gc), dim3(256), 0, c->stream, L->n1c, L->d_Trow, L->d_Tcol,
+                       L->d_Tval, rr, rcv);
+  else
+    hipLaunchKernelGGL((k_csr<float, T, LPR, false>), dim3(gc), dim3(256), 0, c->stream, L->n1c, L->d_Trow, L->d_Tcol,
+                       L->d_Tval32, rr, rcv);
+  MHD_HIP(hipGetLastError());
+  if (cc->coarse) {
+    if ((rc = vcycle<T>(cc, rcv, ec, prec))) return rc;
+  } else {
+    MHD_HIP(hipMemcpyAsync(SC->r1, rcv, sizeof(T) * L->n1c, hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = inner_solve<T>(cc, prec))) return rc;
+    MHD_HIP(hipMemcpyAsync(ec, SC->x1, sizeof(T) * L->n1c, hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (prec == 0)
+    hipLaunchKernelGGL((k_csr<double, T, LPR, true>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow, L->d_Pcol,
+                       L->d_Pval, ec, y);
+  else
+    hipLaunchKernelGGL((k_csr<float, T, LPR, true>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow, L->d_Pcol,
+                       L->d_Pval32, ec, y);
+  resid();
+  MHD_HIP(hipGetLastError());
+  // post-smoothing accumulates into y
+  if ((rc = patch_as<T>(c, rr, y, prec, L->omega, true))) return rc;
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class T>
+int apply_m(mhd_ctx *c, const T *r, T *y, int prec) {
+  return c->coarse ? vcycle<T>(c, r, y, prec) : patch_as<T>(c, r, y, prec);
+}
+
+// x1 ~ A11^-1 r1 by kin steps of right-preconditioned flexible GMRES from x = 0.
 template <class T>
 int inner_solve(mhd_ctx *c, int prec) {
   Solver *S = c->solver;
   const int64_t n1 = S->n1;
   T *V1 = (T *)S->V1, *Z1 = (T *)S->Z1, *w1 = (T *)S->w1, *x1 = (T *)S->x1, *r1 = (T *)S->r1;
   int rc;
-  if (S->kin == 0) return patch_as<T>(c, r1, x1, prec);
+  if (S->kin == 0) return apply_m<T>(c, r1, x1, prec);
   double beta;
   if ((rc = vnorm(c, r1, n1, &beta))) return rc;
   hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(n1)), dim3(TB), 0, c->stream, n1, (T)0, x1);
@@ -614,7 +712,7 @@ int inner_solve(mhd_ctx *c, int prec) {
   std::vector<double> H((S->kin + 1) * ldh, 0.0), h(S->kin + 1), y;
   int k = 0;
   for (int j = 0; j < S->kin; ++j) {
-    if ((rc = patch_as<T>(c, V1 + j * n1, Z1 + j * n1, prec))) return rc;
+    if ((rc = apply_m<T>(c, V1 + j * n1, Z1 + j * n1, prec))) return rc;
     if ((rc = spmv_a11<T>(c, Z1 + j * n1, w1, prec))) return rc;
     double nrm;
     if ((rc = arnoldi<T>(c, V1, n1, j + 1, w1, n1, h.data(), &nrm))) return rc;
@@ -631,10 +729,44 @@ int inner_solve(mhd_ctx *c, int prec) {
   return 0;
 }
 
+// z = P^-T v for the adjoint: P^T = [[A11^T, 0, 0], [A12^T, S_p, 0], [A13^T, 0, S_phi]], so
+// z1 = A11^-T v1 first, then z_p = S_p^-1 (v_p - A12^T z1), z_phi = S_phi^-1 (v_phi - A13^T z1).
+// c->d_val holds A^T here; its p / phi rows only have [u,j] columns.
+int precond_t(mhd_ctx *c, const double *v, double *z, int prec) {
+  Solver *S = c->solver;
+  int rc;
+  const int64_t n = S->n;
+  if (prec == 0) {
+    MHD_HIP(hipMemcpyAsync(S->r1, v, sizeof(double) * S->n1, hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = inner_solve<double>(c, 0))) return rc;
+    MHD_HIP(hipMemcpyAsync(z, S->x1, sizeof(double) * S->n1, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, v,
+                       (float *)S->r1);
+    if ((rc = inner_solve<float>(c, prec))) return rc;
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1,
+                       (const float *)S->x1, z);
+  }
+  // t1[p, phi rows] = v - A^T z (full rows; only the [u,j] part of z is read)
+  constexpr int LPR = 16;
+  const int64_t nr = n - S->n1;
+  const int grid = (int)((nr * LPR + 255) / 256);
+  hipLaunchKernelGGL((k_csr<double, double, LPR, false>), dim3(grid), dim3(256), 0, c->stream, nr,
+                     c->d_rowptr + S->n1, c->d_colind, c->d_val, z, S->t1 + S->n1);
+  hipLaunchKernelGGL((k_scale_copy<double>), dim3(nblocks(nr)), dim3(TB), 0, c->stream, nr, -1.0, S->t1 + S->n1,
+                     S->t1 + S->n1);
+  hipLaunchKernelGGL((k_axpy<double>), dim3(nblocks(nr)), dim3(TB), 0, c->stream, nr, 1.0, v + S->n1, S->t1 + S->n1);
+  hipLaunchKernelGGL((k_schur<double>), dim3(nblocks((int64_t)c->ncells * 12)), dim3(TB), 0, c->stream, c->ncells,
+                     S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, S->t1, z);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
 // z = P^-1 v (FP64 in/out)
 int precond(mhd_ctx *c, const double *v, double *z, int prec) {
   Solver *S = c->solver;
   int rc;
+  if (S->transposed) return precond_t(c, v, z, prec);
   hipLaunchKernelGGL((k_schur<double>), dim3(nblocks((int64_t)c->ncells * 12)), dim3(TB), 0, c->stream, c->ncells,
                      S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, v, z);
   // rhs1 = v1 - A12 z_p - A13 z_phi (FP64)
@@ -660,11 +792,47 @@ int precond(mhd_ctx *c, const double *v, double *z, int prec) {
   return 0;
 }
 
-// Per-Jacobian set-up: Schur scalings, FP32 copy, patch factorisation.
-int precond_setup(mhd_ctx *c, int prec) {
+int solver_init(mhd_ctx *c, const mhd_solver_opts *o);
+
+int coarse_alloc(mhd_ctx *c, int prec) {
+  CoarseLink *L = c->coarse;
+  const int tp = prec == 0 ? 0 : 1;
+  if (L->tprec == tp) return 0;
+  void *ptrs[] = {L->rr, L->rc, L->ec};
+  for (void *p : ptrs)
+    if (p) MHD_HIP(hipFree(p));
+  const size_t es = tp == 0 ? sizeof(double) : sizeof(float);
+  MHD_HIP(hipMalloc(&L->rr, es * (L->n1 + 1)));
+  MHD_HIP(hipMalloc(&L->rc, es * (L->n1c + 1)));
+  MHD_HIP(hipMalloc(&L->ec, es * (L->n1c + 1)));
+  L->tprec = tp;
+  return 0;
+}
+
+// Per-Jacobian set-up: Schur scalings, FP32 copy, patch factorisation; on
+// every coarser level the u state is injected from `d_state` and the [u,j]
+// block is re-assembled with the same kernels (rediscretisation).
+int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
   Solver *S = c->solver;
   int rc;
   if ((rc = inner_alloc(c, prec))) return rc;
+  if (c->coarse) {
+    CoarseLink *L = c->coarse;
+    mhd_ctx *cc = L->ctx;
+    if ((rc = coarse_alloc(c, prec))) return rc;
+    cc->phys = c->phys;
+    hipLaunchKernelGGL(k_inject, dim3(nblocks(cc->n)), dim3(TB), 0, c->stream, cc->n, L->nuc, L->d_inj, d_state,
+                       cc->d_x);
+    MHD_HIP(hipGetLastError());
+    if ((rc = mhd_launch_jacobian(cc, cc->d_x, cc->d_val, c->stream))) {
+      c->err = "coarse level: " + cc->err;
+      return rc;
+    }
+    if ((rc = precond_setup(cc, prec, cc->d_x))) {
+      c->err = "coarse level: " + cc->err;
+      return rc;
+    }
+  }
   const double ap = -1.0 / (c->phys.beta + c->phys.zeta_u);
   const double aphi = -1.0 / (1.0 + c->phys.zeta_j);
   if (ap != S->alpha_p || aphi != S->alpha_phi) {
@@ -791,7 +959,101 @@ void mhd_solver_free(mhd_ctx *c) {
   c->solver = nullptr;
 }
 
+void mhd_coarse_free(mhd_ctx *c) {
+  if (!c || !c->coarse) return;
+  CoarseLink *L = c->coarse;
+  void *ptrs[] = {L->d_Prow, L->d_Trow, L->d_Pcol, L->d_Tcol, L->d_Pval, L->d_Tval, L->d_Pval32, L->d_Tval32,
+                  L->d_inj, L->rr, L->rc, L->ec};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  if (L->ctx) mhd_destroy(L->ctx);
+  delete L;
+  c->coarse = nullptr;
+}
+
 extern "C" {
+
+int mhd_attach_coarse(mhd_ctx *c, mhd_ctx *coarse, int64_t nrows, int64_t ncols, const int64_t *P_rowptr,
+                      const int32_t *P_col, const double *P_val, const int32_t *inj_u, int32_t coarse_iters,
+                      double smoother_weight) {
+  if (!(smoother_weight > 0.0 && smoother_weight <= 1.0)) {
+    if (c) c->err = "mhd_attach_coarse: smoother_weight must be in (0, 1]";
+    return -1;
+  }
+  if (!c || !coarse || !P_rowptr || !P_col || !P_val || !inj_u) return -1;
+  if (c->coarse) {
+    c->err = "a coarse level is already attached";
+    return -1;
+  }
+  const int64_t n1 = (int64_t)c->nfree[0] + c->nfree[2];
+  const int64_t n1c = (int64_t)coarse->nfree[0] + coarse->nfree[2];
+  if (nrows != n1 || ncols != n1c || coarse->device != c->device || coarse->block_style != 1) {
+    c->err = "mhd_attach_coarse: P must be (nfree[u]+nfree[j]) fine x coarse, same device, block numbering";
+    return -1;
+  }
+  const int64_t nnzP = P_rowptr[nrows];
+  for (int64_t k = 0; k < nnzP; ++k)
+    if (P_col[k] < 0 || P_col[k] >= ncols) {
+      c->err = "mhd_attach_coarse: P column out of range";
+      return -1;
+    }
+  for (int64_t i = 0; i < coarse->nfree[0]; ++i)
+    if (inj_u[i] < 0 || inj_u[i] >= c->nfree[0]) {
+      c->err = "mhd_attach_coarse: injection index out of range";
+      return -1;
+    }
+  MHD_HIP(hipSetDevice(c->device));
+  CoarseLink *L = new CoarseLink();
+  L->n1 = n1;
+  L->n1c = n1c;
+  L->nuc = coarse->nfree[0];
+  L->coarse_iters = std::max(1, std::min(coarse_iters, 120));
+  L->omega = smoother_weight;
+  // transpose on the host
+  std::vector<int64_t> trow(ncols + 1, 0);
+  for (int64_t k = 0; k < nnzP; ++k) trow[P_col[k] + 1]++;
+  for (int64_t i = 0; i < ncols; ++i) trow[i + 1] += trow[i];
+  std::vector<int32_t> tcol(nnzP);
+  std::vector<double> tval(nnzP);
+  {
+    std::vector<int64_t> fill(trow.begin(), trow.end() - 1);
+    for (int64_t i = 0; i < nrows; ++i)
+      for (int64_t k = P_rowptr[i]; k < P_rowptr[i + 1]; ++k) {
+        const int64_t q = fill[P_col[k]]++;
+        tcol[q] = (int32_t)i;
+        tval[q] = P_val[k];
+      }
+  }
+  std::vector<float> pv32(P_val, P_val + nnzP), tv32(tval.begin(), tval.end());
+  auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+    MHD_HIP(hipMalloc(dst, std::max<size_t>(bytes, 8)));
+    if (bytes) MHD_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+  };
+  int rc = 0;
+  if (!rc) rc = up((void **)&L->d_Prow, P_rowptr, sizeof(int64_t) * (nrows + 1));
+  if (!rc) rc = up((void **)&L->d_Pcol, P_col, sizeof(int32_t) * nnzP);
+  if (!rc) rc = up((void **)&L->d_Pval, P_val, sizeof(double) * nnzP);
+  if (!rc) rc = up((void **)&L->d_Pval32, pv32.data(), sizeof(float) * nnzP);
+  if (!rc) rc = up((void **)&L->d_Trow, trow.data(), sizeof(int64_t) * (ncols + 1));
+  if (!rc) rc = up((void **)&L->d_Tcol, tcol.data(), sizeof(int32_t) * nnzP);
+  if (!rc) rc = up((void **)&L->d_Tval, tval.data(), sizeof(double) * nnzP);
+  if (!rc) rc = up((void **)&L->d_Tval32, tv32.data(), sizeof(float) * nnzP);
+  if (!rc) rc = up((void **)&L->d_inj, inj_u, sizeof(int32_t) * L->nuc);
+  L->ctx = coarse;
+  c->coarse = L;
+  if (rc) return rc;
+  // the coarse level runs on the fine level's stream
+  MHD_HIP(hipStreamSynchronize(coarse->stream));
+  if (coarse->own_stream) MHD_HIP(hipStreamDestroy(coarse->stream));
+  coarse->stream = c->stream;
+  coarse->own_stream = false;
+  if (coarse->coarse) {  // re-point the whole chain
+    for (mhd_ctx *q = coarse->coarse->ctx; q; q = q->coarse ? q->coarse->ctx : nullptr) q->stream = c->stream;
+  }
+  if (c->solver) mhd_solver_free(c);  // re-initialised with the chain on the next solve
+  return 0;
+}
 
 void mhd_default_solver_opts(mhd_solver_opts *o) {
   std::memset(o, 0, sizeof(*o));
@@ -817,7 +1079,7 @@ int mhd_precond_setup(mhd_ctx *c, const mhd_solver_opts *o) {
                     c->solver->kin != std::max(0, std::min(o->inner_iters, 120))))
     mhd_solver_free(c);
   if ((rc = solver_init(c, o))) return rc;
-  if ((rc = precond_setup(c, o->precision))) return rc;
+  if ((rc = precond_setup(c, o->precision, c->d_x))) return rc;
   MHD_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -832,7 +1094,7 @@ int mhd_smoother_apply_d(mhd_ctx *c, const double *d_r, double *d_y, void *strea
   Solver *S = c->solver;
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   hipLaunchKernelGGL((k_patch_apply<double, double>), dim3(S->npatch), dim3(256), 0, s, S->npatch, S->PS, S->d_psize,
-                     S->d_pidx, S->d_pA, d_r, d_y);
+                     S->d_pidx, S->d_pA, d_r, d_y, 1.0);
   MHD_HIP(hipGetLastError());
   return 0;
 }
@@ -849,6 +1111,86 @@ int mhd_smoother_info(mhd_ctx *c, int64_t *npatch, int64_t *patch_dim, int64_t *
     *patch_entries = t;
   }
   return 0;
+}
+
+int mhd_dresidual_dparam(mhd_ctx *c, const double *x_free, int32_t param, double *out) {
+  if (!c || !x_free || !out) return -1;
+  if (param != 0 && param != 1) {
+    c->err = "mhd_dresidual_dparam: param must be 0 (gamma) or 1 (cw)";
+    return -1;
+  }
+  MHD_HIP(hipSetDevice(c->device));
+  MHD_HIP(hipMemcpyAsync(c->d_x, x_free, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+  int rc = mhd_launch_residual_mode(c, c->d_x, c->d_r, c->stream, param == 0 ? 1 : 2);
+  if (rc) return rc;
+  MHD_HIP(hipMemcpyAsync(out, c->d_r, sizeof(double) * c->n, hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mhd_adjoint_solve(mhd_ctx *c, const double *x_state, const double *rhs, double *lambda,
+                      const mhd_solver_opts *o, mhd_solve_info *info) {
+  if (!c || !x_state || !rhs || !lambda || !o || !info) return -1;
+  std::memset(info, 0, sizeof(*info));
+  if (o->precision < 0 || o->precision > 2) {
+    c->err = "precision must be 0, 1 or 2";
+    return -1;
+  }
+  MHD_HIP(hipSetDevice(c->device));
+  int rc;
+  // the adjoint runs single-level: detach a coarse chain for the duration
+  CoarseLink *chain = c->coarse;
+  c->coarse = nullptr;
+  if (c->solver) mhd_solver_free(c);
+  rc = solver_init(c, o);
+  if (rc) {
+    c->coarse = chain;
+    return rc;
+  }
+  Solver *S = c->solver;
+  const int64_t n = c->n;
+  double *d_valT = nullptr;
+  auto finish = [&](int code) {
+    if (d_valT) {
+      std::swap(c->d_val, d_valT);
+      (void)hipFree(d_valT);
+    }
+    if (c->solver) c->solver->transposed = false;
+    mhd_solver_free(c);
+    c->coarse = chain;
+    return code;
+  };
+  MHD_HIP(hipMemcpyAsync(c->d_x, x_state, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  double t = now();
+  if ((rc = mhd_launch_jacobian(c, c->d_x, c->d_val, c->stream))) return finish(rc);
+  if (hipMalloc(&d_valT, sizeof(double) * std::max<int64_t>(c->nnz, 1)) != hipSuccess) {
+    c->err = "out of device memory for A^T";
+    d_valT = nullptr;
+    return finish(-2);
+  }
+  if ((rc = mhd_launch_transpose_values(c, c->d_val, d_valT, c->stream))) return finish(rc);
+  std::swap(c->d_val, d_valT);  // the solver now sees A^T; d_valT holds A
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  info->t_jacobian = now() - t;
+  t = now();
+  S->transposed = true;
+  if ((rc = precond_setup(c, o->precision, c->d_x))) return finish(rc);
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  info->t_setup = now() - t;
+  t = now();
+  MHD_HIP(hipMemcpyAsync(S->b, rhs, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  double bn;
+  if ((rc = vnorm(c, S->b, n, &bn))) return finish(rc);
+  int kits = 0;
+  rc = fgmres(c, S->b, S->dx, o->ls_rtol * bn, o->max_krylov, o->precision, &kits, o->verbose);
+  if (rc < 0) return finish(rc);
+  MHD_HIP(hipMemcpyAsync(lambda, S->dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  MHD_HIP(hipStreamSynchronize(c->stream));
+  info->t_solve = now() - t;
+  info->krylov_iters = kits;
+  info->converged = rc == 0;
+  info->residual0 = bn;
+  return finish(rc);
 }
 
 int mhd_newton_solve(mhd_ctx *c, double *x_host, const mhd_solver_opts *o, mhd_solve_info *info) {
@@ -888,7 +1230,7 @@ int mhd_newton_solve(mhd_ctx *c, double *x_host, const mhd_solver_opts *o, mhd_s
     MHD_HIP(hipStreamSynchronize(c->stream));
     info->t_jacobian += now() - t;
     t = now();
-    if ((rc = precond_setup(c, o->precision))) return rc;
+    if ((rc = precond_setup(c, o->precision, c->d_x))) return rc;
     MHD_HIP(hipStreamSynchronize(c->stream));
     info->t_setup += now() - t;
     t = now();

@@ -71,6 +71,7 @@ class Spaces:
     face_dof: np.ndarray       # (n_faces,) first field-local j id of the face's 4 dofs
     cell_verts: np.ndarray = None  # (ncells, 8) topological vertex ids
     style: str = "consecutive"
+    bc_spec: dict = None           # the u / j Dirichlet tags and thin walls these tables were built from
     thin_wall_faces: List[dict] = dataclasses.field(default_factory=list)
 
     @property
@@ -220,6 +221,7 @@ def setup_fe_spaces(mesh: CartesianMesh, u_tags, j_tags, thin_walls: Sequence[di
         face_dof=face_dof,
         cell_verts=vert_id.astype(np.int32),
         style=style,
+        bc_spec=dict(u_tags=u_tags, j_tags=j_tags, thin_walls=list(thin_walls)),
     )
     # ---- thin-wall boundary faces -------------------------------------------
     for tw in thin_walls:

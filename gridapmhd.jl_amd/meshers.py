@@ -277,6 +277,10 @@ def hunt_add_tags(mesh: CartesianMesh) -> None:
     mesh.add_tag_from_tags("Ha_walls", list(range(1, 21)) + [23, 24])
     mesh.add_tag_from_tags("side_walls", list(range(1, 21)) + [25, 26])
     mesh.add_tag_from_tags("walls", list(range(1, 21)) + [23, 24, 25, 26])
+    # duct ends of a non-periodic z-duct (channel-style inlet/outlet, src/Applications/channel.jl:251-271)
+    mesh.add_tag_from_tags("inlet", [21])
+    mesh.add_tag_from_tags("outlet", [22])
+    mesh.add_tag_from_tags("ends", [21, 22])
 
 
 def hunt_generate_base_mesh(nc, L=1.0, tw=0.0, Ha=10.0, kmap_x=1, kmap_y=1, BL_adapted=True,

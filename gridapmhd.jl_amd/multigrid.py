@@ -1,0 +1,197 @@
+"""Mesh hierarchies and [u, j] prolongations for the geometric coarse-grid
+correction of the [u, j] block solve.
+
+The hierarchy semi-coarsens x and y by 2 and keeps z, the refinement pattern
+of the reference's GMG hierarchy (``nrefs = (2, 2, 1)``,
+src/Meshers/hunt_mesher.jl:140-155; solver src/Solvers/gmg.jl:323-355).
+Coarse nodes are taken from the fine mesh, so the coarse cells are unions of
+fine cells and the fine spaces contain the coarse ones.  The prolongation
+P (fine [u; j] free dofs x coarse [u; j] free dofs) is the exact embedding:
+every fine dof functional is applied to every coarse basis function.
+  * u (Q2, nodal): coarse Q2 basis evaluated at the fine node.
+  * j (RT1, nodal flux at Gauss points): the coarse field pushed forward with
+    its Piola map, evaluated at the fine dof point, pulled back with the fine
+    cell's Piola map, axis component of the fine dof.
+"""
+from __future__ import annotations
+
+from typing import List, Tuple
+
+import numpy as np
+import scipy.sparse as sps
+
+from . import fespaces
+from .meshers import CartesianMesh
+
+G2 = fespaces.G2
+
+
+def coarsen_mesh(mesh: CartesianMesh) -> CartesianMesh:
+    """Coarse mesh: every second node in x and y, all nodes in z."""
+    nx, ny, nz = mesh.nc
+    if nx % 2 or ny % 2:
+        raise ValueError("x/y cell counts must be even to coarsen")
+    X = mesh.node_xyz.reshape(nz + 1, ny + 1, nx + 1, 3)
+    Xc = X[:, ::2, ::2, :].reshape(-1, 3).copy()
+    cm = CartesianMesh((nx // 2, ny // 2, nz), mesh.domain, mesh.isperiodic, Xc, dict(mesh.tags))
+    return cm
+
+
+def _q2_1d(t):
+    return np.stack([(2 * t - 1) * (t - 1), 4 * t * (1 - t), t * (2 * t - 1)], -1)
+
+
+def _lin_g(t):
+    g0, g1 = G2
+    return np.stack([(g1 - t) / (g1 - g0), (t - g0) / (g1 - g0)], -1)
+
+
+def q2_basis(xi):
+    """(N, 27) Q2 shape values, node a = i + 3j + 9k."""
+    Lx, Ly, Lz = _q2_1d(xi[:, 0]), _q2_1d(xi[:, 1]), _q2_1d(xi[:, 2])
+    return np.einsum("nk,nj,ni->nkji", Lz, Ly, Lx).reshape(-1, 27)
+
+
+def rt_basis(xi):
+    """(N, 36, 3) nodal RT1 reference fields (same local order as the kernels)."""
+    L = [_q2_1d(xi[:, a]) for a in range(3)]
+    G = [_lin_g(xi[:, a]) for a in range(3)]
+    out = np.zeros((xi.shape[0], 36, 3))
+    for r in range(36):
+        if r < 24:
+            f, k = divmod(r, 4)
+            d = 2 - f // 2
+            m = 2 if f % 2 else 0
+        else:
+            d, k = divmod(r - 24, 4)
+            m = 1
+        t1, t2 = {0: (1, 2), 1: (0, 2), 2: (0, 1)}[d]
+        out[:, r, d] = L[d][:, m] * G[t1][:, k % 2] * G[t2][:, k // 2]
+    return out
+
+
+def _locate(coarse: CartesianMesh, fine: CartesianMesh, fine_cells: np.ndarray, fine_xi: np.ndarray):
+    """Coarse cell and coarse reference coordinates of fine reference points
+    (nested meshes: the fine cell (i, j, k) lies in coarse cell (i//2, j//2, k))."""
+    nx, ny, _ = fine.nc
+    fi = fine_cells % nx
+    fj = (fine_cells // nx) % ny
+    fk = fine_cells // (nx * ny)
+    cnx, cny = coarse.nc[0], coarse.nc[1]
+    cc = fi // 2 + cnx * (fj // 2 + cny * fk)
+    xi = fine_xi.copy()
+    xi[:, 0] = 0.5 * (fine_xi[:, 0] + (fi % 2))
+    xi[:, 1] = 0.5 * (fine_xi[:, 1] + (fj % 2))
+    return cc, xi
+
+
+def prolongation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> Tuple[sps.csr_matrix, np.ndarray]:
+    """P (n1_fine x n1_coarse, CSR) on the [u; j] blocks (badia2024 numbering),
+    and the injection map of coarse free u dofs onto fine free u dofs."""
+    assert sp_f.style == "badia2024" and sp_c.style == "badia2024"
+    fine, coarse = sp_f.mesh, sp_c.mesh
+    n1f = sp_f.nfree["u"] + sp_f.nfree["j"]
+    n1c = sp_c.nfree["u"] + sp_c.nfree["j"]
+    cdf = sp_f.cell_dofs.astype(np.int64)
+    cdc = sp_c.cell_dofs.astype(np.int64)
+    rows, cols, vals = [], [], []
+    # ---- u: one representative (cell, local node) per fine free node
+    ids = cdf[:, 0:27]
+    cells, loc = np.nonzero(ids >= 0)
+    g = ids[cells, loc]
+    _, first = np.unique(g, return_index=True)
+    cells, loc, g = cells[first], loc[first], g[first]
+    xi_f = fespaces.Q2_OFFSETS[loc] / 2.0
+    cc, xi_c = _locate(coarse, fine, cells, xi_f)
+    N = q2_basis(xi_c)                                   # (n, 27)
+    for comp in range(3):
+        gfine = cdf[cells, comp * 27 + loc]              # fine free id of this component
+        gcoarse = cdc[cc][:, comp * 27:(comp + 1) * 27]  # (n, 27)
+        m = gcoarse >= 0
+        r = np.broadcast_to(gfine[:, None], gcoarse.shape)[m]
+        rows.append(r); cols.append(gcoarse[m]); vals.append(N[m])
+    # ---- j: one representative (cell, local dof) per fine free j dof
+    off_j = 85
+    ids = cdf[:, off_j:off_j + 36]
+    cells, loc = np.nonzero(ids >= 0)
+    g = ids[cells, loc]
+    _, first = np.unique(g, return_index=True)
+    cells, loc, g = cells[first], loc[first], g[first]
+    pts, axis = fespaces._rt_dof_points()
+    xi_f = pts[loc]
+    cc, xi_c = _locate(coarse, fine, cells, xi_f)
+    # Piola maps of the fine and coarse cells at the point
+    Jf = _jac_at(fine.cell_xyz()[cells], xi_f)
+    Jc = _jac_at(coarse.cell_xyz()[cc], xi_c)
+    detf, detc = np.linalg.det(Jf), np.linalg.det(Jc)
+    # fine dof = axis comp of det_f Jf^-1 (Jc psi_c / det_c)
+    Mmat = np.einsum("n,nla,nab,n->nlb", detf, np.linalg.inv(Jf), Jc, 1.0 / detc)   # (n, 3, 3)
+    psi = rt_basis(xi_c)                                                             # (n, 36, 3)
+    row_axis = Mmat[np.arange(len(cells)), axis[loc], :]                             # (n, 3)
+    W = np.einsum("nb,nrb->nr", row_axis, psi)                                       # (n, 36)
+    sgn_c = np.ones((len(cells), 36))
+    sgn_c[:, :24] = np.repeat(sp_c.rt_sign[cc].astype(float), 4, axis=1)
+    sgn_f = np.where(loc < 24, sp_f.rt_sign[cells, np.minimum(loc, 23) // 4].astype(float), 1.0)
+    W = W * sgn_c * sgn_f[:, None]
+    gcoarse = cdc[cc][:, off_j:off_j + 36]
+    m = (gcoarse >= 0) & (np.abs(W) > 1e-14)
+    rows.append(np.broadcast_to(g[:, None], gcoarse.shape)[m]); cols.append(gcoarse[m]); vals.append(W[m])
+    r = np.concatenate(rows); c = np.concatenate(cols); v = np.concatenate(vals)
+    keep = np.abs(v) > 1e-14
+    P = sps.csr_matrix((v[keep], (r[keep], c[keep])), shape=(n1f, n1c))
+    P.sum_duplicates()
+    # ---- state injection: coarse free u dof -> fine free u dof at the same node
+    inj = _u_injection(sp_f, sp_c)
+    return P, inj
+
+
+def _jac_at(cell_xyz, xi):
+    """Jacobians of the trilinear maps of cells[k] at points xi[k] -> (n, 3, 3)."""
+    n = xi.shape[0]
+    dN = np.empty((n, 8, 3))
+    for v in range(8):
+        b = [(v >> a) & 1 for a in range(3)]
+        f = [xi[:, a] if b[a] else 1.0 - xi[:, a] for a in range(3)]
+        df = [np.full(n, 1.0) if b[a] else np.full(n, -1.0) for a in range(3)]
+        dN[:, v, 0] = df[0] * f[1] * f[2]
+        dN[:, v, 1] = f[0] * df[1] * f[2]
+        dN[:, v, 2] = f[0] * f[1] * df[2]
+    return np.einsum("nvl,nvi->nil", dN, cell_xyz)
+
+
+def _u_injection(sp_f, sp_c):
+    fine, coarse = sp_f.mesh, sp_c.mesh
+    cdf, cdc = sp_f.cell_dofs.astype(np.int64), sp_c.cell_dofs.astype(np.int64)
+    nfx, nfy = fine.nc[0], fine.nc[1]
+    cnx, cny = coarse.nc[0], coarse.nc[1]
+    # coarse cell (I, J, K) corner -> fine cell (2I, 2J, K); coarse node (i, j, k) in {0,1,2}^3 ->
+    # fine lattice offset (2i, 2j, k) from the fine corner cell, i.e. fine cell (2I + i//... )
+    cc = np.arange(coarse.ncells)
+    I, J, K = cc % cnx, (cc // cnx) % cny, cc // (cnx * cny)
+    inj = np.full(sp_c.nfree["u"], -1, dtype=np.int64)
+    for a in range(27):
+        i, j, k = a % 3, (a // 3) % 3, a // 9
+        # fine lattice point (2I*2 + 2i, ...): fine cell index and local node
+        fi = 2 * I + (i if i < 2 else 1)
+        fj = 2 * J + (j if j < 2 else 1)
+        li = 0 if i < 2 else 2
+        lj = 0 if j < 2 else 2
+        fcell = fi + nfx * (fj + nfy * K)
+        lf = li + 3 * lj + 9 * k
+        for comp in range(3):
+            gc = cdc[cc, comp * 27 + a]
+            gf = cdf[fcell, comp * 27 + lf]
+            m = gc >= 0
+            inj[gc[m]] = gf[m]
+    assert (inj >= 0).all()
+    return inj
+
+
+def hierarchy(sp: fespaces.Spaces, nlevels: int, u_tags, j_tags, thin_walls) -> List[fespaces.Spaces]:
+    """[fine, coarse1, ...] spaces with the same boundary conditions."""
+    out = [sp]
+    mesh = sp.mesh
+    for _ in range(nlevels - 1):
+        mesh = coarsen_mesh(mesh)
+        out.append(fespaces.setup_fe_spaces(mesh, u_tags, j_tags, thin_walls, style=sp.style))
+    return out

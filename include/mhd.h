@@ -151,6 +151,34 @@ int mhd_spmv_d(mhd_ctx *ctx, const double *d_x, double *d_y, void *stream);
 int mhd_newton_solve(mhd_ctx *ctx, double *x_free, const mhd_solver_opts *opts, mhd_solve_info *info);
 void mhd_default_solver_opts(mhd_solver_opts *opts);
 
+/* Adjoint / sensitivities (no counterpart in the reference, which has no
+ * adjoint; they serve the differentiation configuration).
+ * mhd_dresidual_dparam: out = dR/dtheta at x for theta = gamma (param 0: the
+ * Lorentz term -int (j x B).v) or cw (param 1: int_G tau (s.n) n.(grad j).n
+ * over every thin-wall face).  dR/dHa = (2 Ha / Re) dR/dgamma in the :cfd scaling.
+ * mhd_adjoint_solve: lambda = A(x)^-T rhs with the transposed Badia2024
+ * preconditioner (lower block-triangular), single level.  Then
+ * dJ/dtheta = dJ/dtheta|_explicit - lambda . dR/dtheta. */
+int mhd_dresidual_dparam(mhd_ctx *ctx, const double *x_free, int32_t param, double *out);
+int mhd_adjoint_solve(mhd_ctx *ctx, const double *x_state, const double *rhs, double *lambda,
+                      const mhd_solver_opts *opts, mhd_solve_info *info);
+
+/* Attach a coarser level to the [u,j]-block solve (geometric coarse-grid
+ * correction; the x,y semi-coarsened hierarchy of the reference's GMG,
+ * src/Meshers/hunt_mesher.jl:140-155, src/Solvers/gmg.jl:323-355).  `coarse` is a
+ * context on the coarse mesh with the same parameters and block numbering;
+ * P (CSR, nrows = nfree[u]+nfree[j] of ctx, ncols = the same of coarse) maps
+ * coarse [u,j] values to fine ones; inj_u[i] is the fine u dof carrying the
+ * value of coarse u dof i (state injection for the coarse Jacobian).  ctx takes
+ * ownership of coarse (destroyed with ctx; do not destroy it separately).  The
+ * [u,j] smoother of mhd_newton_solve then becomes a V-cycle whose vertex-star
+ * Schwarz pre/post-smoothing is damped by smoother_weight (0, 1] (the patches
+ * overlap up to 8-fold; 0.2 is a good value); on the coarsest level
+ * coarse_iters Schwarz-GMRES steps are taken. */
+int mhd_attach_coarse(mhd_ctx *ctx, mhd_ctx *coarse, int64_t nrows, int64_t ncols, const int64_t *P_rowptr,
+                      const int32_t *P_col, const double *P_val, const int32_t *inj_u, int32_t coarse_iters,
+                      double smoother_weight);
+
 /* Preconditioner set-up for the last assembled Jacobian: the numerical_setup!
  * of the block solver (src/Solvers/badia2024.jl:20-31) -- Schur scalings,
  * vertex-star patch extraction and batched LU.  opts->precision selects the

@@ -62,3 +62,26 @@ def test_newton_solve_matches_direct(name, precision, orc_mod):
         assert err["all"] < bar, err
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("nc,levels", [((8, 8, 3), 2), ((16, 16, 3), 3)])
+def test_multilevel_solve_matches_direct(nc, levels, precision, orc_mod):
+    """The [u,j] V-cycle (semi-coarsened levels attached through mhd_attach_coarse)."""
+    import gridapmhd_amd as g
+    _say(f"[solver] multilevel nc={nc} levels={levels} precision={precision}")
+    sp, phys = cases.duct_thin_wall_case(nc, Ha=30.0, Re=10.0, cw=0.02, convection="newton", style="badia2024")
+    orc = orc_mod.Oracle(cases.problem_dict(sp, phys))
+    x_ref = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), maxiter=20, rtol=1e-13, atol=1e-15,
+                                  pins=orc_mod.null_pins(sp.nfree))
+    ctx = g._lib.context_with_levels(sp, dict(phys, zeta_u=1.0, zeta_j=1.0), levels=levels, coarse_iters=10)
+    try:
+        o = _opts(g, precision)
+        o.inner_iters = 3
+        x, info, rc = ctx.newton_solve(np.zeros(sp.ndofs), o)
+        err = cases.field_errors(sp, x, x_ref)
+        _say(f"  newton={info.newton_iters} krylov={info.krylov_iters} rc={rc} err={err['all']:.2e}")
+        assert rc == 0 and info.converged
+        assert err["all"] < (1e-10 if precision == 0 else 1e-8), err
+    finally:
+        ctx.close()
