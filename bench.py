@@ -120,7 +120,9 @@ def main():
     ap.add_argument("--asm-reps", type=int, default=10)
     ap.add_argument("--mixed", type=int, default=1, choices=[1, 2], help="mixed mode: 1 FP32 / 2 BF16 patch blocks")
     ap.add_argument("--restart", type=int, default=100)
-    ap.add_argument("--inner", type=int, default=8)
+    ap.add_argument("--inner", type=int, default=3, help="flexible-GMRES steps of the inexact [u,j] solve")
+    ap.add_argument("--levels", type=int, default=4, help="levels of the [u,j] V-cycle (1 = Schwarz only)")
+    ap.add_argument("--coarse-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -135,7 +137,7 @@ def main():
     sp, phys = build_problem(cfg)
     log(f"mesh+spaces {cfg['nc']}: ndofs={sp.ndofs} ({time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
-    ctx = g._lib.context_from_spaces(sp, phys)
+    ctx = g._lib.context_with_levels(sp, phys, levels=args.levels, coarse_iters=args.coarse_iters)
     t_ctx = time.perf_counter() - t0
     log(f"context: nnz={ctx.nnz} ({t_ctx:.1f}s)")
     dev = torch.device("cuda:0")
@@ -156,8 +158,15 @@ def main():
         ctx.residual_d(x.data_ptr(), r.data_ptr(), sh)
     torch.cuda.synchronize()
     t_jac = time_events(lambda: ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh), args.asm_reps, stream)
+    val_col = val.clone()
+    ctx.set_assembly_mode(0)
+    ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh)
+    t_jac_atomic = time_events(lambda: ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh), args.asm_reps, stream)
+    atomic_diff = float((val - val_col).abs().max() / val_col.abs().max())
+    ctx.set_assembly_mode(1)
     t_res = time_events(lambda: ctx.residual_d(x.data_ptr(), r.data_ptr(), sh), args.asm_reps, stream)
-    log(f"jacobian {t_jac * 1e3:.3f} ms ({ncells / t_jac:.3e} cells/s), residual {t_res * 1e3:.3f} ms")
+    log(f"jacobian {t_jac * 1e3:.3f} ms colour classes ({ncells / t_jac:.3e} cells/s), "
+        f"{t_jac_atomic * 1e3:.3f} ms atomic (max diff {atomic_diff:.1e}), residual {t_res * 1e3:.3f} ms")
     nnz = ctx.nnz
     jac_bytes = nnz * 8 + ncells * (129 * (8 + 4) + 24 * 8)
     # ---- SpMV --------------------------------------------------------------
@@ -252,16 +261,19 @@ def main():
         "config": {"workload": f"thin-wall duct {'x'.join(map(str, cfg['nc']))} hex, Q2/dP1/RT1/dQ1, "
                                f"Ha={cfg['Ha']:g}, Re={cfg['Re']:g}, cw={cfg['cw']:g}, Newton convection",
                    "ndofs": sp.ndofs, "ncells": ncells, "nnz": nnz, "zeta": [1.0, 1.0],
-                   "restart": args.restart, "inner_iters": args.inner},
+                   "restart": args.restart, "inner_iters": args.inner, "levels": args.levels,
+                   "coarse_iters": args.coarse_iters},
         "fp64_newton_step_s": round(fp64["newton_step_s"], 4),
         "mixed_over_fp64_speedup": round(fp64["newton_step_s"] / mixed["newton_step_s"], 3),
         "rel_err": {"fp64": fp64["rel_err"], "mixed": mixed["rel_err"]},
         "solves": results,
         "asm_cells_per_s": round(ncells / t_jac, 1),
         "asm_jacobian_ms": round(t_jac * 1e3, 4),
+        "asm_jacobian_atomic_ms": round(t_jac_atomic * 1e3, 4),
         "asm_residual_ms": round(t_res * 1e3, 4),
         "residual_cells_per_s": round(ncells / t_res, 1),
-        "roofline": roof("jacobian assembly (memset + k_cell_jacobian + k_face_jacobian)", jac_bytes, t_jac),
+        "roofline": roof("jacobian assembly, colour classes (memset + k_cell_jacobian + k_face_jacobian)",
+                         jac_bytes, t_jac),
         "roofline_kernels": [roof("k_spmv_seg/k_spmv fp64 CSR", spmv_bytes, t_spmv),
                              roof("k_patch_apply<double> vertex-star Schwarz", sm_bytes, t_sm)],
         "reference_solve": dict(seconds=round(t_ref, 2), newton=info_ref.newton_iters,

@@ -58,6 +58,7 @@ EXPORTS = {
     "mhd_set_params": (ctypes.c_int, [c_void_p, ctypes.POINTER(MhdParams)]),
     "mhd_set_dirichlet": (ctypes.c_int, [c_void_p, c_void_p]),
     "mhd_sizes": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
+    "mhd_set_assembly_mode": (ctypes.c_int, [c_void_p, c_int32]),
     "mhd_pattern": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "mhd_residual": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "mhd_jacobian": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
@@ -207,6 +208,10 @@ class Context:
         out = np.empty((self.ncells, 6))
         self._check(self.lib.mhd_cell_integrals(self.h, _ptr(x), _ptr(out)), "mhd_cell_integrals")
         return out
+
+    def set_assembly_mode(self, mode: int):
+        """1 = colour classes (deterministic, default), 0 = atomic scatter."""
+        self._check(self.lib.mhd_set_assembly_mode(self.h, int(mode)), "mhd_set_assembly_mode")
 
     def set_params(self, phys):
         self.params = make_params(phys)

@@ -147,6 +147,48 @@ int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, co
   if ((rc = dev_upload(c, &c->d_colind, c->h_colind.data(), (size_t)c->nnz))) return rc;
   MHD_HIP(hipMalloc(&c->d_rank, sizeof(uint16_t) * (size_t)c->ncells * mhd::NENT));
   if ((rc = mhd_build_rank_map(c))) return rc;
+  // greedy colouring of the cell / shared-dof graph
+  {
+    std::vector<int64_t> ptr(n + 1, 0);
+    for (int64_t k = 0; k < ntab; ++k)
+      if (spaces->cell_dofs[k] >= 0) ptr[spaces->cell_dofs[k] + 1]++;
+    for (int64_t i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int32_t> cells_of(ptr[n]);
+    {
+      std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+      for (int32_t cc = 0; cc < c->ncells; ++cc)
+        for (int l = 0; l < mhd::NDOF; ++l) {
+          const int32_t g = spaces->cell_dofs[(int64_t)cc * mhd::NDOF + l];
+          if (g >= 0) cells_of[fill[g]++] = cc;
+        }
+    }
+    std::vector<int32_t> color(c->ncells, -1);
+    std::vector<int32_t> seen(64, -1);
+    int32_t ncol = 0;
+    for (int32_t cc = 0; cc < c->ncells; ++cc) {
+      for (int l = 0; l < mhd::NDOF; ++l) {
+        const int32_t g = spaces->cell_dofs[(int64_t)cc * mhd::NDOF + l];
+        if (g < 0) continue;
+        for (int64_t q = ptr[g]; q < ptr[g + 1]; ++q) {
+          const int32_t o = color[cells_of[q]];
+          if (o >= 0) {
+            if (o >= (int32_t)seen.size()) seen.resize(o + 1, -1);
+            seen[o] = cc;
+          }
+        }
+      }
+      int32_t k = 0;
+      while (k < (int32_t)seen.size() && seen[k] == cc) ++k;
+      color[cc] = k;
+      ncol = std::max(ncol, k + 1);
+    }
+    c->color_ptr.assign(ncol + 1, 0);
+    for (int32_t cc = 0; cc < c->ncells; ++cc) c->color_ptr[color[cc] + 1]++;
+    for (int32_t k = 0; k < ncol; ++k) c->color_ptr[k + 1] += c->color_ptr[k];
+    std::vector<int32_t> sorted(c->ncells), fill(c->color_ptr.begin(), c->color_ptr.end() - 1);
+    for (int32_t cc = 0; cc < c->ncells; ++cc) sorted[fill[color[cc]]++] = cc;
+    if ((rc = dev_upload(c, &c->d_color_cells, sorted.data(), sorted.size()))) return rc;
+  }
   MHD_HIP(hipMalloc(&c->d_val, sizeof(double) * std::max<int64_t>(c->nnz, 1)));
   MHD_HIP(hipMalloc(&c->d_x, sizeof(double) * std::max<int64_t>(n, 1)));
   MHD_HIP(hipMalloc(&c->d_r, sizeof(double) * std::max<int64_t>(n, 1)));
@@ -160,7 +202,8 @@ void mhd_destroy(mhd_ctx *c) {
   mhd_solver_free(c);
   mhd_coarse_free(c);
   void *ptrs[] = {c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw,
-                  c->d_rowptr, c->d_colind, c->d_rank, c->d_val, c->d_tperm, c->d_ref, c->d_fref, c->d_x, c->d_r};
+                  c->d_rowptr, c->d_colind, c->d_rank, c->d_val, c->d_tperm, c->d_ref, c->d_fref, c->d_x, c->d_r,
+                  c->d_color_cells};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -181,6 +224,16 @@ int mhd_set_dirichlet(mhd_ctx *c, const double *dir_vals) {
   }
   MHD_HIP(hipSetDevice(c->device));
   MHD_HIP(hipMemcpy(c->d_dir, dir_vals, sizeof(double) * c->ndir, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int mhd_set_assembly_mode(mhd_ctx *c, int32_t mode) {
+  if (!c) return -1;
+  if (mode != 0 && mode != 1) {
+    c->err = "assembly mode must be 0 (atomic) or 1 (colour classes)";
+    return -1;
+  }
+  c->assembly_mode = mode;
   return 0;
 }
 

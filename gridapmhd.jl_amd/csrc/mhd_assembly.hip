@@ -410,6 +410,15 @@ __device__ double res_entry(const CellLDS &S, const RefTables *__restrict__ T, c
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
+// Accumulate into the CSR values: atomically, or -- when the launch covers one
+// colour class, so that no two cells of the launch share a dof -- with a plain
+// read-modify-write (deterministic, no atomic traffic).
+template <bool ATOMIC>
+__device__ inline void add_val(double *p, double v) {
+  if (ATOMIC) atomicAdd(p, v);
+  else *p += v;
+}
+
 __device__ inline d4_t mfma16(double a, double b, d4_t c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -417,7 +426,7 @@ __device__ inline d4_t mfma16(double a, double b, d4_t c) {
 // Scatter-add one 16x16 MFMA result tile.  Lane l holds rows l/16 + 4 r
 // (r = 0..3), column l%16 of the tile (v_mfma_f64_16x16x4f64 output layout); (row0, col0) are the local dof offsets of the tile,
 // `ent(i, j)` maps tile-local (i, j) to the canonical entry of the rank map.
-template <class EntFn>
+template <bool ATOMIC, class EntFn>
 __device__ inline void scatter_tile(const CellLDS &S, const d4_t &acc, double scale, int row0, int nrow, int col0,
                                     int ncol, const uint16_t *__restrict__ rk, double *__restrict__ val, EntFn ent) {
   const int lane = threadIdx.x & 63;
@@ -431,7 +440,7 @@ __device__ inline void scatter_tile(const CellLDS &S, const d4_t &acc, double sc
     if (i >= nrow) continue;
     const int lr = row0 + i;
     if (S.g[lr] < 0) continue;
-    atomicAdd(&val[S.rp[lr] + rk[ent(i, j)]], scale * acc[r]);
+    add_val<ATOMIC>(&val[S.rp[lr] + rk[ent(i, j)]], scale * acc[r]);
   }
 }
 
@@ -443,6 +452,7 @@ __device__ inline void scatter_tile(const CellLDS &S, const d4_t &acc, double sc
 //     (-sigma (e_c phi_a x B).psi_r = +sigma phi_a (psi_r x B)_c, so ju = -(sigma/gamma) uj^T)
 //   jj[r][s]   = sum_q W (psi_r . psi_s + zeta_j div psi_r div psi_s)                    (K = 27 q x 4)
 // The small blocks up / pu (81 x 4) and jphi / phij (36 x 8) run on the vector ALUs.
+template <bool ATOMIC>
 __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const uint16_t *__restrict__ rk,
                                     double *__restrict__ val) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -481,7 +491,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
           acc = mfma16(A, Bv, acc);
         }
         const int r0 = c * 27 + 16 * ti, c0 = d * 27 + 16 * tj;
-        scatter_tile(S, acc, 1.0, r0, min(16, 27 - 16 * ti), c0, min(16, 27 - 16 * tj), rk, val,
+        scatter_tile<ATOMIC>(S, acc, 1.0, r0, min(16, 27 - 16 * ti), c0, min(16, 27 - 16 * tj), rk, val,
                      [&](int i, int j) { return (r0 + i) * ROWS_U_LEN + (c0 + j); });
       }
     } else if (job < 13) {
@@ -497,7 +507,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
         acc = mfma16(A, Bv, acc);
       }
       const int r0 = 16 * ti, s0 = 16 * tj;
-      scatter_tile(S, acc, 1.0, OFF_J + r0, min(16, 36 - r0), OFF_J + s0, min(16, 36 - s0), rk, val,
+      scatter_tile<ATOMIC>(S, acc, 1.0, OFF_J + r0, min(16, 36 - r0), OFF_J + s0, min(16, 36 - s0), rk, val,
                    [&](int i, int j) { return E_J + (r0 + i) * ROWS_J_LEN + NU + (s0 + j); });
     } else {
       const int t = job - 13, c = t / 6, tt = t % 6, ti = tt / 3, tj = tt % 3;
@@ -516,7 +526,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
       const int a0 = 16 * ti, r0 = 16 * tj;
       const int na = min(16, 27 - a0), nr = min(16, 36 - r0);
       // uj = -gamma X   (u row c*27 + a, j column r)
-      scatter_tile(S, acc, -ph.gamma, c * 27 + a0, na, OFF_J + r0, nr, rk, val,
+      scatter_tile<ATOMIC>(S, acc, -ph.gamma, c * 27 + a0, na, OFF_J + r0, nr, rk, val,
                    [&](int i, int j) { return (c * 27 + a0 + i) * ROWS_U_LEN + OFF_J + r0 + j; });
       // ju = +sigma X^T (j row r, u column c*27 + a): lane holds X[a0+i][r0+j] -> entry (j row r0+j, u col a0+i)
       {
@@ -529,7 +539,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
             if (i >= na) continue;
             const int lc = c * 27 + a0 + i;
             if (S.g[lc] < 0) continue;
-            atomicAdd(&val[S.rp[lr] + rk[E_J + (r0 + j) * ROWS_J_LEN + lc]], ph.sigma * acc[rr]);
+            add_val<ATOMIC>(&val[S.rp[lr] + rk[E_J + (r0 + j) * ROWS_J_LEN + lc]], ph.sigma * acc[rr]);
           }
         }
       }
@@ -542,31 +552,37 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
       if (S.g[row] < 0 || S.g[OFF_P + k] < 0) continue;
       double acc = 0.0;
       for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.p1[q][k] * S.gphi[q][na][ca];
-      atomicAdd(&val[S.rp[row] + rk[row * ROWS_U_LEN + OFF_P + k]], acc);                 // - dp div v
-      atomicAdd(&val[S.rp[OFF_P + k] + rk[E_P + k * ROWS_P_LEN + row]], acc);             // - div du q
+      add_val<ATOMIC>(&val[S.rp[row] + rk[row * ROWS_U_LEN + OFF_P + k]], acc);           // - dp div v
+      add_val<ATOMIC>(&val[S.rp[OFF_P + k] + rk[E_P + k * ROWS_P_LEN + row]], acc);       // - div du q
     } else {
       const int tt = t - NU * 4, r = tt >> 3, v = tt & 7;
       if (S.g[OFF_J + r] < 0 || S.g[OFF_PHI + v] < 0) continue;
       double acc = 0.0;
       for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.q1[q][v] * S.dpsi[q][r];
-      atomicAdd(&val[S.rp[OFF_J + r] + rk[E_J + r * ROWS_J_LEN + NU + NJ + v]], ph.sigma * acc);  // - sigma dphi div s
-      atomicAdd(&val[S.rp[OFF_PHI + v] + rk[E_PHI + v * ROWS_PHI_LEN + r]], acc);                 // - div dj phi_t
+      add_val<ATOMIC>(&val[S.rp[OFF_J + r] + rk[E_J + r * ROWS_J_LEN + NU + NJ + v]], ph.sigma * acc);  // - sigma dphi div s
+      add_val<ATOMIC>(&val[S.rp[OFF_PHI + v] + rk[E_PHI + v * ROWS_PHI_LEN + r]], acc);           // - div dj phi_t
     }
   }
 }
 
-__global__ __launch_bounds__(WG, 2) void k_cell_jacobian(int ncells, const double *__restrict__ xyz,
-                                                      const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
-                                                      const double *__restrict__ dir, const double *__restrict__ x,
-                                                      const RefTables *__restrict__ T, const int64_t *__restrict__ rowptr,
-                                                      const uint16_t *__restrict__ rank, double *__restrict__ val,
-                                                      MhdPhys ph) {
+// cells[i] (or i when cells == nullptr) for i < ncells; ATOMIC = false needs
+// a colour class: no two listed cells share a dof.
+template <bool ATOMIC>
+__global__ __launch_bounds__(WG, 2) void k_cell_jacobian(int ncells, const int32_t *__restrict__ cells,
+                                                         const double *__restrict__ xyz,
+                                                         const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
+                                                         const double *__restrict__ dir, const double *__restrict__ x,
+                                                         const RefTables *__restrict__ T,
+                                                         const int64_t *__restrict__ rowptr,
+                                                         const uint16_t *__restrict__ rank, double *__restrict__ val,
+                                                         MhdPhys ph) {
   __shared__ CellLDS S;
-  for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
+  for (int i = blockIdx.x; i < ncells; i += gridDim.x) {
+    const int c = cells ? cells[i] : i;
     stage_cell(S, c, xyz, dofs, sign, dir, x, T, rowptr, ph, ph.zeta_u != 0.0);
     if (threadIdx.x < 3) S.B[threadIdx.x] = ph.B[threadIdx.x];
     __syncthreads();
-    cell_jacobian_tiles(S, ph, rank + (int64_t)c * NENT, val);
+    cell_jacobian_tiles<ATOMIC>(S, ph, rank + (int64_t)c * NENT, val);
     __syncthreads();
   }
 }
@@ -852,11 +868,19 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restri
 
 int mhd_launch_jacobian(mhd_ctx *c, const double *d_x, double *d_val, hipStream_t s) {
   MHD_HIP(hipMemsetAsync(d_val, 0, sizeof(double) * c->nnz, s));
-  if (c->ncells > 0) {
-    const int grid = c->ncells;
-    hipLaunchKernelGGL(k_cell_jacobian, dim3(grid), dim3(WG), 0, s, c->ncells, c->d_xyz, c->d_dofs, c->d_sign,
-                       c->d_dir, d_x, c->d_ref, c->d_rowptr, c->d_rank, d_val, c->phys);
+  if (c->ncells > 0 && (c->assembly_mode == 0 || c->color_ptr.size() < 2)) {
+    hipLaunchKernelGGL((k_cell_jacobian<true>), dim3(c->ncells), dim3(WG), 0, s, c->ncells, (const int32_t *)nullptr,
+                       c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, d_x, c->d_ref, c->d_rowptr, c->d_rank, d_val, c->phys);
     MHD_HIP(hipGetLastError());
+  } else if (c->ncells > 0) {
+    for (size_t k = 0; k + 1 < c->color_ptr.size(); ++k) {
+      const int n = c->color_ptr[k + 1] - c->color_ptr[k];
+      if (n == 0) continue;
+      hipLaunchKernelGGL((k_cell_jacobian<false>), dim3(n), dim3(WG), 0, s, n, c->d_color_cells + c->color_ptr[k],
+                         c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, d_x, c->d_ref, c->d_rowptr, c->d_rank, d_val,
+                         c->phys);
+      MHD_HIP(hipGetLastError());
+    }
   }
   if (c->nfaces > 0) {
     hipLaunchKernelGGL(k_face_jacobian, dim3(c->nfaces), dim3(64), 0, s, c->nfaces, c->d_xyz, c->d_dofs, c->d_sign,

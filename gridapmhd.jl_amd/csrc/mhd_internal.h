@@ -99,6 +99,10 @@ struct mhd_ctx {
   std::vector<double> h_xyz;       // [ncells][24]
   int32_t nverts = 0;
   int32_t block_style = 0;
+  // cell colouring: no two cells of a class share a dof (deterministic assembly)
+  int32_t assembly_mode = 1;          // 0 atomic scatter, 1 colour classes (default)
+  std::vector<int32_t> color_ptr;     // [ncolors + 1]
+  int32_t *d_color_cells = nullptr;   // cells sorted by colour
   // solver workspace (allocated lazily)
   struct Solver *solver = nullptr;
   // coarser level of the [u,j] block solve (owned), see mhd_attach_coarse

@@ -112,6 +112,25 @@ function Gridap.Algebra.solve!(x::AbstractVector, ns::HIPNewtonSolver, op::Grida
   (info = info, rc = rc)
 end
 
+"""
+    attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=10, ω=0.2)
+
+Chains coarser contexts (built from `hunt_generate_mesh_hierarchy`-style
+x,y-semi-coarsened models) onto `ctx` through `mhd_attach_coarse`; each
+prolongation is a `SparseMatrixCSR` fine x coarse on the [u; j] dofs.
+"""
+function attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=10, ω=0.2)
+  for lev in length(coarse_ctxs):-1:1
+    fine = lev == 1 ? ctx : coarse_ctxs[lev-1]
+    P = prolongations[lev]
+    rc = ccall((:mhd_attach_coarse, libmhd), Cint,
+               (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Ptr{Int64}, Ptr{Int32}, Ptr{Float64}, Ptr{Int32}, Int32, Float64),
+               fine, coarse_ctxs[lev], size(P, 1), size(P, 2), P.rowptr .- 1, Int32.(P.colval .- 1), P.nzval,
+               Int32.(injections[lev] .- 1), coarse_iters, ω)
+    check(rc, fine, "mhd_attach_coarse")
+  end
+end
+
 function hip_postpro(cache, out)
   out[:nls_iters] = cache.info.newton_iters
   out[:ls_iters] = cache.info.krylov_iters

@@ -116,6 +116,11 @@ const char *mhd_last_error(const mhd_ctx *ctx);
 int mhd_set_params(mhd_ctx *ctx, const mhd_params *params);
 int mhd_set_dirichlet(mhd_ctx *ctx, const double *dir_vals);
 
+/* Jacobian scatter: 1 (default) = one launch per colour class of cells that
+ * share no dof, plain read-modify-write, bitwise reproducible; 0 = one launch,
+ * FP64 atomic adds (summation order not reproducible). */
+int mhd_set_assembly_mode(mhd_ctx *ctx, int32_t mode);
+
 /* Sizes of the system. */
 int mhd_sizes(mhd_ctx *ctx, int64_t *nfree, int64_t *nnz);
 

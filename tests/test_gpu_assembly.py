@@ -81,3 +81,29 @@ def test_manufactured_residual_vanishes():
         assert np.max(np.abs(r)) < 1e-12
     finally:
         ctx.close()
+
+
+def test_colour_and_atomic_scatter_agree():
+    """Colour-class assembly (default) is bitwise reproducible and equals the
+    atomic scatter up to summation order."""
+    import gridapmhd_amd as g
+    _say("[assembly] colour vs atomic, hunt 6x6x3")
+    sp, phys = cases.hunt_case(6, Ha=50.0)
+    rng = np.random.default_rng(7)
+    x = 0.1 * rng.standard_normal(sp.ndofs)
+    ctx = g._lib.context_from_spaces(sp, phys)
+    try:
+        a1 = ctx.jacobian(x)
+        a2 = ctx.jacobian(x)
+        assert np.array_equal(a1, a2)
+        ctx.set_assembly_mode(0)
+        b = ctx.jacobian(x)
+        ctx.set_assembly_mode(1)
+        scale = np.max(np.abs(a1))
+        err = np.max(np.abs(a1 - b)) / scale
+        _say(f"  max diff / max entry {err:.3e}")
+        assert err < 1e-14
+        with pytest.raises(g._lib.MhdError):
+            ctx.set_assembly_mode(5)
+    finally:
+        ctx.close()
