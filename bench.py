@@ -10,9 +10,12 @@ walls.  Synthetic inputs only.
 
 A "step" is one Newton step: residual assembly + Jacobian assembly +
 preconditioner set-up + block-preconditioned FGMRES solve + update, from the
-zero initial guess.  `value` is the mixed-precision Newton-step time; the FP64
-form is reported beside it, both with their relative solution error against a
-tightly converged FP64 solve of the same problem.  Assembly throughput
+zero initial guess (Newton tolerance 1e-10, linear tolerance 1e-11; the
+[u,j] block is solved inexactly by 3 flexible-GMRES steps preconditioned by a
+V-cycle over x/y semi-coarsened levels with vertex-star Schwarz smoothing and
+an exact coarsest solve).  `value` is the mixed-precision Newton-step time;
+the FP64 form is reported beside it, both with their relative solution error
+against an FP64 solve through a different hierarchy.  Assembly throughput
 (cells/s) is timed with HIP events on the stream the kernels run on, with
 inputs resident in HBM.
 """
@@ -119,10 +122,11 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--asm-reps", type=int, default=10)
     ap.add_argument("--mixed", type=int, default=1, choices=[1, 2], help="mixed mode: 1 FP32 / 2 BF16 patch blocks")
-    ap.add_argument("--restart", type=int, default=100)
+    ap.add_argument("--restart", type=int, default=200)
     ap.add_argument("--inner", type=int, default=3, help="flexible-GMRES steps of the inexact [u,j] solve")
-    ap.add_argument("--levels", type=int, default=4, help="levels of the [u,j] V-cycle (1 = Schwarz only)")
-    ap.add_argument("--coarse-iters", type=int, default=10)
+    ap.add_argument("--levels", default="auto", help="levels of the [u,j] V-cycle (1 = Schwarz only; auto: 5 at C2)")
+    ap.add_argument("--coarse-iters", type=int, default=0, help="0: exact coarsest solve")
+    ap.add_argument("--weight", type=float, default=0.2, help="Schwarz smoother damping in the V-cycle")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -137,7 +141,9 @@ def main():
     sp, phys = build_problem(cfg)
     log(f"mesh+spaces {cfg['nc']}: ndofs={sp.ndofs} ({time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
-    ctx = g._lib.context_with_levels(sp, phys, levels=args.levels, coarse_iters=args.coarse_iters)
+    levels = g.multigrid.auto_levels(sp.mesh) if args.levels == "auto" else int(args.levels)
+    ctx = g._lib.context_with_levels(sp, phys, levels=levels, coarse_iters=args.coarse_iters,
+                                     smoother_weight=args.weight)
     t_ctx = time.perf_counter() - t0
     log(f"context: nnz={ctx.nnz} ({t_ctx:.1f}s)")
     dev = torch.device("cuda:0")
@@ -158,15 +164,18 @@ def main():
         ctx.residual_d(x.data_ptr(), r.data_ptr(), sh)
     torch.cuda.synchronize()
     t_jac = time_events(lambda: ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh), args.asm_reps, stream)
-    val_col = val.clone()
-    ctx.set_assembly_mode(0)
-    ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh)
-    t_jac_atomic = time_events(lambda: ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh), args.asm_reps, stream)
-    atomic_diff = float((val - val_col).abs().max() / val_col.abs().max())
-    ctx.set_assembly_mode(1)
+    val_ref = val.clone()
+    t_modes, diff_modes = {}, {}
+    for mode in (1, 0):     # colour classes, atomic scatter
+        ctx.set_assembly_mode(mode)
+        ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh)
+        t_modes[mode] = time_events(lambda: ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh), args.asm_reps, stream)
+        diff_modes[mode] = float((val - val_ref).abs().max() / val_ref.abs().max())
+    ctx.set_assembly_mode(2)
     t_res = time_events(lambda: ctx.residual_d(x.data_ptr(), r.data_ptr(), sh), args.asm_reps, stream)
-    log(f"jacobian {t_jac * 1e3:.3f} ms colour classes ({ncells / t_jac:.3e} cells/s), "
-        f"{t_jac_atomic * 1e3:.3f} ms atomic (max diff {atomic_diff:.1e}), residual {t_res * 1e3:.3f} ms")
+    log(f"jacobian {t_jac * 1e3:.3f} ms row assembly ({ncells / t_jac:.3e} cells/s), "
+        f"{t_modes[1] * 1e3:.3f} ms colour classes, {t_modes[0] * 1e3:.3f} ms atomic "
+        f"(max diffs {diff_modes[1]:.1e} {diff_modes[0]:.1e}), residual {t_res * 1e3:.3f} ms")
     nnz = ctx.nnz
     jac_bytes = nnz * 8 + ncells * (129 * (8 + 4) + 24 * 8)
     # ---- SpMV --------------------------------------------------------------
@@ -179,26 +188,40 @@ def main():
     # ---- solves ------------------------------------------------------------
     x0 = np.zeros(sp.ndofs)
 
-    def opts(prec, rtol, ls_rtol, maxk):
-        return ctx.default_opts(precision=prec, rtol=rtol, atol=1e-14, ls_rtol=ls_rtol, max_krylov=maxk,
-                                restart=args.restart, inner_iters=args.inner, max_newton=8)
+    def opts(prec, rtol, ls_rtol, maxk, max_newton=8):
+        return ctx.default_opts(precision=prec, rtol=rtol, atol=1e-16, ls_rtol=ls_rtol, max_krylov=maxk,
+                                restart=args.restart, inner_iters=args.inner, max_newton=max_newton)
 
+    # Reference: FP64 through a different preconditioner (one level fewer) to a
+    # 3e-11 relative nonlinear residual (the FP64 floor of the residual
+    # evaluation is near 1e-11).  The fully developed duct solution is axial and
+    # z-independent, so the convective term vanishes on it and one Newton step
+    # from zero, solved to that accuracy, already reaches it.
     t0 = time.perf_counter()
-    x_ref, info_ref, rc_ref = ctx.newton_solve(x0, opts(0, 1e-11, 1e-11, 3000))
+    ref_levels = max(1, levels - 1)
+    ctx_ref = g._lib.context_with_levels(sp, phys, levels=ref_levels, coarse_iters=args.coarse_iters,
+                                         smoother_weight=args.weight)
+    o_ref = ctx_ref.default_opts(precision=0, rtol=3e-11, atol=1e-16, ls_rtol=1e-12, max_krylov=800, restart=400,
+                                 inner_iters=args.inner, max_newton=2)
+    x_ref, info_ref, rc_ref = ctx_ref.newton_solve(x0, o_ref)
+    ctx_ref.close()
     t_ref = time.perf_counter() - t0
-    log(f"reference FP64 solve: newton={info_ref.newton_iters} krylov={info_ref.krylov_iters} "
-        f"|R|={info_ref.residual:.2e} rc={rc_ref} ({t_ref:.1f}s)")
+    log(f"reference FP64 solve ({ref_levels} levels): newton={info_ref.newton_iters} krylov={info_ref.krylov_iters} "
+        f"|R0|={info_ref.residual0:.3e} |R|={info_ref.residual:.2e} rc={rc_ref} ({t_ref:.1f}s)")
 
     def rel_err(xs):
-        ps = sp.field_slices()["p"]
+        # p and phi are defined up to constants (no pressure or current Dirichlet data)
+        fs = sp.field_slices()
         a, b = xs.copy(), x_ref.copy()
-        a[ps] -= a[ps].mean()
-        b[ps] -= b[ps].mean()
+        for f in ("p", "phi"):
+            a[fs[f]] -= a[fs[f]].mean()
+            b[fs[f]] -= b[fs[f]].mean()
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
     results = {}
     for label, prec in (("fp64", 0), ("mixed", args.mixed)):
-        o = opts(prec, 1e-8, 1e-9, 3000)
+        # Newton tolerance 1e-10 with the linear tolerance at a tenth of it
+        o = opts(prec, 1e-10, 1e-11, 3000)
         for _ in range(args.warmup):
             ctx.newton_solve(x0, o)
         t_tot, n_tot, k_tot = 0.0, 0, 0
@@ -213,24 +236,30 @@ def main():
             k_tot += info.krylov_iters
             parts += [info.t_residual, info.t_jacobian, info.t_setup, info.t_solve]
         err = rel_err(xs)
-        results[label] = dict(newton_step_s=t_tot / max(n_tot, 1), newton_iters=n_tot / args.steps,
+        results[label] = dict(newton_step_s=t_tot / max(n_tot, 1), solve_s=t_tot / args.steps,
+                              newton_iters=n_tot / args.steps,
                               krylov_iters=k_tot / args.steps, rel_err=err, converged=bool(info.converged),
                               phase_s=dict(zip(("residual", "jacobian", "setup", "solve"),
                                                (parts / max(n_tot, 1)).round(4).tolist())))
         log(f"{label}: {results[label]}")
 
-    # ---- smoother kernel (vertex-star additive Schwarz, FP64 blocks) --------
+    # ---- smoother kernel (vertex-star additive Schwarz), FP64 and FP32 blocks --
     ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh)
     torch.cuda.synchronize()
-    ctx.precond_setup(opts(0, 1e-8, 1e-9, 10))
-    npatch, pdim, pent = ctx.smoother_info()
     n1 = sp.nfree["u"] + sp.nfree["j"]
-    rr = torch.rand(n1, dtype=torch.float64, device=dev)
-    yy = torch.zeros(n1, dtype=torch.float64, device=dev)
-    ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh)
-    t_sm = time_events(lambda: ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh), 10, stream)
-    sm_bytes = pent * 8
-    log(f"smoother apply {t_sm * 1e3:.3f} ms ({sm_bytes / t_sm / 1e9:.0f} GB/s), npatch={npatch} dim={pdim}")
+    t_sm = {}
+    for prec, dt in ((0, torch.float64), (args.mixed, torch.float32)):
+        ctx.precond_setup(opts(prec, 1e-8, 1e-9, 10))
+        rr = torch.rand(n1, dtype=dt, device=dev)
+        yy = torch.zeros(n1, dtype=dt, device=dev)
+        ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh)
+        t_sm[prec] = time_events(lambda: ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh), 10, stream)
+    npatch, pdim, pent = ctx.smoother_info()
+    esize = {0: 8, 1: 4, 2: 2}
+    sm_bytes = {p: pent * esize[p] for p in t_sm}
+    log(f"smoother apply FP64 {t_sm[0] * 1e3:.3f} ms ({sm_bytes[0] / t_sm[0] / 1e9:.0f} GB/s), "
+        f"mixed {t_sm[args.mixed] * 1e3:.3f} ms ({sm_bytes[args.mixed] / t_sm[args.mixed] / 1e9:.0f} GB/s), "
+        f"npatch={npatch} dim={pdim}")
 
     cpu = None
     if not args.no_cpu:
@@ -261,23 +290,28 @@ def main():
         "config": {"workload": f"thin-wall duct {'x'.join(map(str, cfg['nc']))} hex, Q2/dP1/RT1/dQ1, "
                                f"Ha={cfg['Ha']:g}, Re={cfg['Re']:g}, cw={cfg['cw']:g}, Newton convection",
                    "ndofs": sp.ndofs, "ncells": ncells, "nnz": nnz, "zeta": [1.0, 1.0],
-                   "restart": args.restart, "inner_iters": args.inner, "levels": args.levels,
-                   "coarse_iters": args.coarse_iters},
+                   "restart": args.restart, "inner_iters": args.inner, "levels": levels,
+                   "coarse_iters": args.coarse_iters, "smoother_weight": args.weight,
+                   "newton_rtol": 1e-10, "linear_rtol": 1e-11},
         "fp64_newton_step_s": round(fp64["newton_step_s"], 4),
         "mixed_over_fp64_speedup": round(fp64["newton_step_s"] / mixed["newton_step_s"], 3),
         "rel_err": {"fp64": fp64["rel_err"], "mixed": mixed["rel_err"]},
         "solves": results,
         "asm_cells_per_s": round(ncells / t_jac, 1),
         "asm_jacobian_ms": round(t_jac * 1e3, 4),
-        "asm_jacobian_atomic_ms": round(t_jac_atomic * 1e3, 4),
+        "asm_jacobian_colour_ms": round(t_modes[1] * 1e3, 4),
+        "asm_jacobian_atomic_ms": round(t_modes[0] * 1e3, 4),
         "asm_residual_ms": round(t_res * 1e3, 4),
         "residual_cells_per_s": round(ncells / t_res, 1),
-        "roofline": roof("jacobian assembly, colour classes (memset + k_cell_jacobian + k_face_jacobian)",
-                         jac_bytes, t_jac),
-        "roofline_kernels": [roof("k_spmv_seg/k_spmv fp64 CSR", spmv_bytes, t_spmv),
-                             roof("k_patch_apply<double> vertex-star Schwarz", sm_bytes, t_sm)],
-        "reference_solve": dict(seconds=round(t_ref, 2), newton=info_ref.newton_iters,
-                                krylov=info_ref.krylov_iters, residual=info_ref.residual),
+        # the dominant kernel of the mixed Newton step: the fine-level smoother
+        "roofline": roof(f"k_patch_apply<{'float' if args.mixed == 1 else 'bf16'}> vertex-star Schwarz, "
+                         f"fine level ({npatch} patches)", sm_bytes[args.mixed], t_sm[args.mixed]),
+        "roofline_kernels": [roof("k_patch_apply<double> vertex-star Schwarz, fine level", sm_bytes[0], t_sm[0]),
+                             roof("k_spmv fp64 CSR", spmv_bytes, t_spmv),
+                             roof("jacobian assembly (k_cell_jacobian + k_row_assemble + k_face_jacobian)",
+                                  jac_bytes, t_jac)],
+        "reference_solve": dict(seconds=round(t_ref, 2), levels=ref_levels, newton=info_ref.newton_iters,
+                                krylov=info_ref.krylov_iters, rel_residual=info_ref.residual / info_ref.residual0),
         "context_setup_s": round(t_ctx, 2),
         "cpu_baseline": cpu,
     }

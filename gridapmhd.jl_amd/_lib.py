@@ -210,7 +210,8 @@ class Context:
         return out
 
     def set_assembly_mode(self, mode: int):
-        """1 = colour classes (deterministic, default), 0 = atomic scatter."""
+        """2 = element buffer + row assembly (default), 1 = colour classes,
+        0 = atomic scatter."""
         self._check(self.lib.mhd_set_assembly_mode(self.h, int(mode)), "mhd_set_assembly_mode")
 
     def set_params(self, phys):
@@ -272,6 +273,7 @@ class Context:
         return self._check(self.lib.mhd_precond_setup(self.h, ctypes.byref(opts)), "mhd_precond_setup")
 
     def smoother_apply_d(self, d_r: int, d_y: int, stream: int = 0):
+        """y += smoother(r); float64 vectors after a precision-0 set-up, float32 otherwise."""
         return self._check(self.lib.mhd_smoother_apply_d(self.h, d_r, d_y, stream or None), "mhd_smoother_apply_d")
 
     def smoother_info(self):
@@ -291,11 +293,14 @@ class Context:
         return x, info, rc
 
 
-def context_with_levels(sp, phys, levels=1, coarse_iters=10, smoother_weight=0.2, device=0) -> Context:
+def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.2, device=0) -> Context:
     """Fine context plus `levels - 1` semi-coarsened levels attached for the
     [u,j] V-cycle (badia2024 numbering, nested x/y coarsening, the same
-    boundary conditions on every level)."""
+    boundary conditions on every level).  `levels="auto"` picks the depth
+    from the mesh; `coarse_iters = 0` solves the coarsest level exactly."""
     from . import fespaces, multigrid
+    if levels == "auto":
+        levels = multigrid.auto_levels(sp.mesh)
     ctx = context_from_spaces(sp, phys, device=device)
     if levels <= 1:
         return ctx

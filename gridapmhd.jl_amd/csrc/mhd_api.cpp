@@ -188,7 +188,39 @@ int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, co
     std::vector<int32_t> sorted(c->ncells), fill(c->color_ptr.begin(), c->color_ptr.end() - 1);
     for (int32_t cc = 0; cc < c->ncells; ++cc) sorted[fill[color[cc]]++] = cc;
     if ((rc = dev_upload(c, &c->d_color_cells, sorted.data(), sorted.size()))) return rc;
+    // row assembly lists: cells_of is already in increasing cell order per dof
+    if ((int64_t)c->ncells * mhd::NDOF >= ((int64_t)1 << 31)) {
+      c->err = "too many cells for the row-assembly index (int32)";
+      return -1;
+    }
+    std::vector<int32_t> crow(n + 1), centry(ptr[n]);
+    for (int64_t i = 0; i <= n; ++i) crow[i] = (int32_t)ptr[i];
+    {
+      std::vector<int64_t> fill2(ptr.begin(), ptr.end() - 1);
+      for (int32_t cc = 0; cc < c->ncells; ++cc)
+        for (int l = 0; l < mhd::NDOF; ++l) {
+          const int32_t g = spaces->cell_dofs[(int64_t)cc * mhd::NDOF + l];
+          if (g >= 0) centry[fill2[g]++] = cc * mhd::NDOF + l;
+        }
+    }
+    if ((rc = dev_upload(c, &c->d_crow, crow.data(), crow.size()))) return rc;
+    if (centry.empty()) centry.push_back(0);
+    if ((rc = dev_upload(c, &c->d_centry, centry.data(), centry.size()))) return rc;
+    // a dof listed twice in one cell (periodic direction one cell thick)
+    for (int32_t cc = 0; cc < c->ncells && !c->dup_dofs; ++cc) {
+      int32_t v[mhd::NDOF];
+      int m = 0;
+      for (int l = 0; l < mhd::NDOF; ++l) {
+        const int32_t g = spaces->cell_dofs[(int64_t)cc * mhd::NDOF + l];
+        if (g >= 0) v[m++] = g;
+      }
+      std::sort(v, v + m);
+      for (int l = 1; l < m; ++l)
+        if (v[l] == v[l - 1]) c->dup_dofs = true;
+    }
   }
+  for (int64_t i = 0; i < n; ++i)
+    c->max_row = std::max<int32_t>(c->max_row, (int32_t)(c->h_rowptr[i + 1] - c->h_rowptr[i]));
   MHD_HIP(hipMalloc(&c->d_val, sizeof(double) * std::max<int64_t>(c->nnz, 1)));
   MHD_HIP(hipMalloc(&c->d_x, sizeof(double) * std::max<int64_t>(n, 1)));
   MHD_HIP(hipMalloc(&c->d_r, sizeof(double) * std::max<int64_t>(n, 1)));
@@ -203,7 +235,7 @@ void mhd_destroy(mhd_ctx *c) {
   mhd_coarse_free(c);
   void *ptrs[] = {c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw,
                   c->d_rowptr, c->d_colind, c->d_rank, c->d_val, c->d_tperm, c->d_ref, c->d_fref, c->d_x, c->d_r,
-                  c->d_color_cells};
+                  c->d_color_cells, c->d_crow, c->d_centry, c->d_ebuf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -229,8 +261,8 @@ int mhd_set_dirichlet(mhd_ctx *c, const double *dir_vals) {
 
 int mhd_set_assembly_mode(mhd_ctx *c, int32_t mode) {
   if (!c) return -1;
-  if (mode != 0 && mode != 1) {
-    c->err = "assembly mode must be 0 (atomic) or 1 (colour classes)";
+  if (mode < 0 || mode > 2) {
+    c->err = "assembly mode must be 0 (atomic), 1 (colour classes) or 2 (element buffer + row assembly)";
     return -1;
   }
   c->assembly_mode = mode;

@@ -410,12 +410,25 @@ __device__ double res_entry(const CellLDS &S, const RefTables *__restrict__ T, c
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
-// Accumulate into the CSR values: atomically, or -- when the launch covers one
-// colour class, so that no two cells of the launch share a dof -- with a plain
-// read-modify-write (deterministic, no atomic traffic).
-template <bool ATOMIC>
-__device__ inline void add_val(double *p, double v) {
-  if (ATOMIC) atomicAdd(p, v);
+// How a cell's Jacobian entries leave the kernel:
+//   SCATTER_ATOMIC  FP64 atomic add into the CSR values;
+//   SCATTER_COLOUR  plain read-modify-write (the launch covers one colour class:
+//                   no two of its cells share a dof);
+//   SCATTER_EBUF    plain store of every canonical entry into the cell's slot of
+//                   the element buffer (summed into CSR rows by k_row_assemble).
+enum { SCATTER_ATOMIC = 0, SCATTER_COLOUR = 1, SCATTER_EBUF = 2 };
+
+// Entry e (canonical index) of local row lr / local column lc with value v.
+template <int MODE>
+__device__ inline void emit(const CellLDS &S, const uint16_t *__restrict__ rk, double *__restrict__ out, int e, int lr,
+                            int lc, double v) {
+  if (MODE == SCATTER_EBUF) {
+    out[e] = v;
+    return;
+  }
+  if (S.g[lr] < 0 || S.g[lc] < 0) return;
+  double *p = &out[S.rp[lr] + rk[e]];
+  if (MODE == SCATTER_ATOMIC) atomicAdd(p, v);
   else *p += v;
 }
 
@@ -426,21 +439,17 @@ __device__ inline d4_t mfma16(double a, double b, d4_t c) {
 // Scatter-add one 16x16 MFMA result tile.  Lane l holds rows l/16 + 4 r
 // (r = 0..3), column l%16 of the tile (v_mfma_f64_16x16x4f64 output layout); (row0, col0) are the local dof offsets of the tile,
 // `ent(i, j)` maps tile-local (i, j) to the canonical entry of the rank map.
-template <bool ATOMIC, class EntFn>
+template <int MODE, class EntFn>
 __device__ inline void scatter_tile(const CellLDS &S, const d4_t &acc, double scale, int row0, int nrow, int col0,
-                                    int ncol, const uint16_t *__restrict__ rk, double *__restrict__ val, EntFn ent) {
+                                    int ncol, const uint16_t *__restrict__ rk, double *__restrict__ out, EntFn ent) {
   const int lane = threadIdx.x & 63;
   const int j = lane & 15;
   if (j >= ncol) return;
-  const int lc = col0 + j;
-  if (S.g[lc] < 0) return;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int i = (lane >> 4) + 4 * r;
     if (i >= nrow) continue;
-    const int lr = row0 + i;
-    if (S.g[lr] < 0) continue;
-    add_val<ATOMIC>(&val[S.rp[lr] + rk[ent(i, j)]], scale * acc[r]);
+    emit<MODE>(S, rk, out, ent(i, j), row0 + i, col0 + j, scale * acc[r]);
   }
 }
 
@@ -452,7 +461,9 @@ __device__ inline void scatter_tile(const CellLDS &S, const d4_t &acc, double sc
 //     (-sigma (e_c phi_a x B).psi_r = +sigma phi_a (psi_r x B)_c, so ju = -(sigma/gamma) uj^T)
 //   jj[r][s]   = sum_q W (psi_r . psi_s + zeta_j div psi_r div psi_s)                    (K = 27 q x 4)
 // The small blocks up / pu (81 x 4) and jphi / phij (36 x 8) run on the vector ALUs.
-template <bool ATOMIC>
+// `out` is the CSR value array (atomic / colour modes) or the cell's element
+// buffer slot (NENT doubles, element-buffer mode).
+template <int MODE>
 __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const uint16_t *__restrict__ rk,
                                     double *__restrict__ val) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -491,7 +502,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
           acc = mfma16(A, Bv, acc);
         }
         const int r0 = c * 27 + 16 * ti, c0 = d * 27 + 16 * tj;
-        scatter_tile<ATOMIC>(S, acc, 1.0, r0, min(16, 27 - 16 * ti), c0, min(16, 27 - 16 * tj), rk, val,
+        scatter_tile<MODE>(S, acc, 1.0, r0, min(16, 27 - 16 * ti), c0, min(16, 27 - 16 * tj), rk, val,
                      [&](int i, int j) { return (r0 + i) * ROWS_U_LEN + (c0 + j); });
       }
     } else if (job < 13) {
@@ -507,7 +518,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
         acc = mfma16(A, Bv, acc);
       }
       const int r0 = 16 * ti, s0 = 16 * tj;
-      scatter_tile<ATOMIC>(S, acc, 1.0, OFF_J + r0, min(16, 36 - r0), OFF_J + s0, min(16, 36 - s0), rk, val,
+      scatter_tile<MODE>(S, acc, 1.0, OFF_J + r0, min(16, 36 - r0), OFF_J + s0, min(16, 36 - s0), rk, val,
                    [&](int i, int j) { return E_J + (r0 + i) * ROWS_J_LEN + NU + (s0 + j); });
     } else {
       const int t = job - 13, c = t / 6, tt = t % 6, ti = tt / 3, tj = tt % 3;
@@ -515,21 +526,28 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
       const double b1 = S.B[c1], b2 = S.B[c2];
       const int a = 16 * ti + li, r = 16 * tj + li;
       const bool av = a < 27, rv = r < 36;
-      d4_t acc = zero;
+      d4_t acc = zero, accT = zero;
 #pragma unroll 1
       for (int st = 0; st < 7; ++st) {
         const int q = 4 * st + lk;
         const double A = (av && q < NQ) ? S.q2[q][a] : 0.0;
         const double Bv = (rv && q < NQ) ? S.W[q] * (S.psi[q][r][c1] * b2 - S.psi[q][r][c2] * b1) : 0.0;
         acc = mfma16(A, Bv, acc);
+        // A and B share the lane -> (index, k) layout, so swapping them gives X^T
+        if (MODE == SCATTER_EBUF) accT = mfma16(Bv, A, accT);
       }
       const int a0 = 16 * ti, r0 = 16 * tj;
       const int na = min(16, 27 - a0), nr = min(16, 36 - r0);
       // uj = -gamma X   (u row c*27 + a, j column r)
-      scatter_tile<ATOMIC>(S, acc, -ph.gamma, c * 27 + a0, na, OFF_J + r0, nr, rk, val,
+      scatter_tile<MODE>(S, acc, -ph.gamma, c * 27 + a0, na, OFF_J + r0, nr, rk, val,
                    [&](int i, int j) { return (c * 27 + a0 + i) * ROWS_U_LEN + OFF_J + r0 + j; });
-      // ju = +sigma X^T (j row r, u column c*27 + a): lane holds X[a0+i][r0+j] -> entry (j row r0+j, u col a0+i)
-      {
+      // ju = +sigma X^T (j row r, u column c*27 + a)
+      if (MODE == SCATTER_EBUF) {
+        // accT lane holds X^T[r0 + i][a0 + j]: rows of the j block, contiguous u columns
+        scatter_tile<MODE>(S, accT, ph.sigma, OFF_J + r0, nr, c * 27 + a0, na, rk, val,
+                           [&](int i, int j) { return E_J + (r0 + i) * ROWS_J_LEN + c * 27 + a0 + j; });
+      } else {
+        // lane holds X[a0+i][r0+j] -> entry (j row r0+j, u col a0+i)
         const int j = lane & 15;
         if (j < nr && S.g[OFF_J + r0 + j] >= 0) {
           const int lr = OFF_J + r0 + j;
@@ -539,7 +557,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
             if (i >= na) continue;
             const int lc = c * 27 + a0 + i;
             if (S.g[lc] < 0) continue;
-            add_val<ATOMIC>(&val[S.rp[lr] + rk[E_J + (r0 + j) * ROWS_J_LEN + lc]], ph.sigma * acc[rr]);
+            emit<MODE>(S, rk, val, E_J + (r0 + j) * ROWS_J_LEN + lc, lr, lc, ph.sigma * acc[rr]);
           }
         }
       }
@@ -549,25 +567,26 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
   for (int t = threadIdx.x; t < NU * 4 + NJ * 8; t += WG) {
     if (t < NU * 4) {
       const int row = t >> 2, k = t & 3, ca = row / 27, na = row % 27;
-      if (S.g[row] < 0 || S.g[OFF_P + k] < 0) continue;
+      if (MODE != SCATTER_EBUF && (S.g[row] < 0 || S.g[OFF_P + k] < 0)) continue;
       double acc = 0.0;
       for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.p1[q][k] * S.gphi[q][na][ca];
-      add_val<ATOMIC>(&val[S.rp[row] + rk[row * ROWS_U_LEN + OFF_P + k]], acc);           // - dp div v
-      add_val<ATOMIC>(&val[S.rp[OFF_P + k] + rk[E_P + k * ROWS_P_LEN + row]], acc);       // - div du q
+      emit<MODE>(S, rk, val, row * ROWS_U_LEN + OFF_P + k, row, OFF_P + k, acc);     // - dp div v
+      emit<MODE>(S, rk, val, E_P + k * ROWS_P_LEN + row, OFF_P + k, row, acc);       // - div du q
     } else {
       const int tt = t - NU * 4, r = tt >> 3, v = tt & 7;
-      if (S.g[OFF_J + r] < 0 || S.g[OFF_PHI + v] < 0) continue;
+      if (MODE != SCATTER_EBUF && (S.g[OFF_J + r] < 0 || S.g[OFF_PHI + v] < 0)) continue;
       double acc = 0.0;
       for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.q1[q][v] * S.dpsi[q][r];
-      add_val<ATOMIC>(&val[S.rp[OFF_J + r] + rk[E_J + r * ROWS_J_LEN + NU + NJ + v]], ph.sigma * acc);  // - sigma dphi div s
-      add_val<ATOMIC>(&val[S.rp[OFF_PHI + v] + rk[E_PHI + v * ROWS_PHI_LEN + r]], acc);           // - div dj phi_t
+      emit<MODE>(S, rk, val, E_J + r * ROWS_J_LEN + NU + NJ + v, OFF_J + r, OFF_PHI + v, ph.sigma * acc);  // - sigma dphi div s
+      emit<MODE>(S, rk, val, E_PHI + v * ROWS_PHI_LEN + r, OFF_PHI + v, OFF_J + r, acc);                   // - div dj phi_t
     }
   }
 }
 
-// cells[i] (or i when cells == nullptr) for i < ncells; ATOMIC = false needs
-// a colour class: no two listed cells share a dof.
-template <bool ATOMIC>
+// cells[i] (or i when cells == nullptr) for i < ncells; SCATTER_COLOUR needs
+// a colour class: no two listed cells share a dof.  SCATTER_EBUF writes cell c's
+// entries to val + c * NENT (val is then the element buffer).
+template <int MODE>
 __global__ __launch_bounds__(WG, 2) void k_cell_jacobian(int ncells, const int32_t *__restrict__ cells,
                                                          const double *__restrict__ xyz,
                                                          const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
@@ -582,7 +601,7 @@ __global__ __launch_bounds__(WG, 2) void k_cell_jacobian(int ncells, const int32
     stage_cell(S, c, xyz, dofs, sign, dir, x, T, rowptr, ph, ph.zeta_u != 0.0);
     if (threadIdx.x < 3) S.B[threadIdx.x] = ph.B[threadIdx.x];
     __syncthreads();
-    cell_jacobian_tiles<ATOMIC>(S, ph, rank + (int64_t)c * NENT, val);
+    cell_jacobian_tiles<MODE>(S, ph, rank + (int64_t)c * NENT, MODE == SCATTER_EBUF ? val + (int64_t)c * NENT : val);
     __syncthreads();
   }
 }
@@ -801,6 +820,84 @@ __global__ __launch_bounds__(64) void k_face_jacobian(int nfaces, const double *
   }
 }
 
+// Canonical entry range [start, start + len) of local row a of a cell.
+__device__ inline void row_segment(int a, int &start, int &len) {
+  if (a < OFF_P) {
+    start = a * ROWS_U_LEN;
+    len = ROWS_U_LEN;
+  } else if (a < OFF_J) {
+    start = E_P + (a - OFF_P) * ROWS_P_LEN;
+    len = ROWS_P_LEN;
+  } else if (a < OFF_PHI) {
+    start = E_J + (a - OFF_J) * ROWS_J_LEN;
+    len = ROWS_J_LEN;
+  } else {
+    start = E_PHI + (a - OFF_PHI) * ROWS_PHI_LEN;
+    len = ROWS_PHI_LEN;
+  }
+}
+
+// Row assembly from the element buffer: one 64-lane wave per CSR row.  The row
+// is accumulated in LDS -- every (cell, local row) holding the row's dof adds
+// its contiguous element-buffer segment at the entries' ranks, in increasing
+// cell order -- and then written once, coalesced.  Every entry of the row is
+// written, so val needs no clearing.  A cell holds a dof at most once, so the
+// ranks of one segment are distinct and its lanes never collide in LDS.
+// All holders of a dof hold it as the same field, so the segments of a row
+// share one length (<= 125 = 2 x 64 lanes); up to RA_BATCH segments are loaded
+// into registers together before they are added in order.
+constexpr int RA_WAVES = 4;
+constexpr int RA_BATCH = 8;
+__global__ __launch_bounds__(64 * RA_WAVES) void k_row_assemble(int64_t n, const int64_t *__restrict__ rowptr,
+                                                                const int32_t *__restrict__ crow,
+                                                                const int32_t *__restrict__ centry,
+                                                                const double *__restrict__ ebuf,
+                                                                const uint16_t *__restrict__ rank,
+                                                                double *__restrict__ val, int maxrow) {
+  extern __shared__ double ra_lds[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double *buf = ra_lds + (size_t)wave * maxrow;
+  for (int64_t row = (int64_t)blockIdx.x * RA_WAVES + wave; row < n; row += (int64_t)gridDim.x * RA_WAVES) {
+    const int64_t b = rowptr[row];
+    const int len = (int)(rowptr[row + 1] - b);
+    const int p0 = crow[row], p1 = crow[row + 1];
+    for (int t = lane; t < len; t += 64) buf[t] = 0.0;
+    __builtin_amdgcn_wave_barrier();
+    for (int pb = p0; pb < p1; pb += RA_BATCH) {
+      double v0[RA_BATCH], v1[RA_BATCH];
+      uint16_t k0[RA_BATCH], k1[RA_BATCH];
+#pragma unroll
+      for (int u = 0; u < RA_BATCH; ++u) {
+        k0[u] = k1[u] = RANK_NONE;
+        v0[u] = v1[u] = 0.0;
+        if (pb + u < p1) {
+          const int32_t ce = centry[pb + u];
+          const int cell = ce / NDOF, a = ce - cell * NDOF;
+          int start, slen;
+          row_segment(a, start, slen);
+          const int64_t base = (int64_t)cell * NENT + start;
+          if (lane < slen) {
+            k0[u] = rank[base + lane];
+            v0[u] = ebuf[base + lane];
+          }
+          if (lane + 64 < slen) {
+            k1[u] = rank[base + lane + 64];
+            v1[u] = ebuf[base + lane + 64];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < RA_BATCH; ++u) {
+        if (k0[u] != RANK_NONE) buf[k0[u]] += v0[u];
+        if (k1[u] != RANK_NONE) buf[k1[u]] += v1[u];
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    for (int t = lane; t < len; t += 64) val[b + t] = buf[t];
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // Rank of every touched cell entry inside its CSR row (binary search).
 __global__ void k_rank_map(int ncells, const int32_t *__restrict__ dofs, const int64_t *__restrict__ rowptr,
                            const int32_t *__restrict__ colind, uint16_t *__restrict__ rank, int *__restrict__ bad) {
@@ -867,18 +964,39 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restri
 }  // namespace
 
 int mhd_launch_jacobian(mhd_ctx *c, const double *d_x, double *d_val, hipStream_t s) {
-  MHD_HIP(hipMemsetAsync(d_val, 0, sizeof(double) * c->nnz, s));
-  if (c->ncells > 0 && (c->assembly_mode == 0 || c->color_ptr.size() < 2)) {
-    hipLaunchKernelGGL((k_cell_jacobian<true>), dim3(c->ncells), dim3(WG), 0, s, c->ncells, (const int32_t *)nullptr,
-                       c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, d_x, c->d_ref, c->d_rowptr, c->d_rank, d_val, c->phys);
+  // a cell holding one dof twice (a periodic direction one cell thick) needs
+  // the atomic scatter: its own entries may coincide
+  int mode = c->dup_dofs ? 0 : c->assembly_mode;
+  if (mode == 2 && sizeof(double) * (size_t)RA_WAVES * c->max_row > 65536) mode = 1;  // row too long for LDS
+  if (mode == 2 && c->ncells > 0) {
+    if (!c->d_ebuf) MHD_HIP(hipMalloc(&c->d_ebuf, sizeof(double) * (size_t)c->ncells * NENT));
+    hipLaunchKernelGGL((k_cell_jacobian<SCATTER_EBUF>), dim3(c->ncells), dim3(WG), 0, s, c->ncells,
+                       (const int32_t *)nullptr, c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, d_x, c->d_ref, c->d_rowptr,
+                       c->d_rank, c->d_ebuf, c->phys);
     MHD_HIP(hipGetLastError());
-  } else if (c->ncells > 0) {
+    if (c->n > 0) {
+      const int64_t blocks = std::min<int64_t>((c->n + RA_WAVES - 1) / RA_WAVES, 1 << 20);
+      const size_t lds = sizeof(double) * (size_t)RA_WAVES * c->max_row;
+      hipLaunchKernelGGL(k_row_assemble, dim3((unsigned)blocks), dim3(64 * RA_WAVES), lds, s, c->n, c->d_rowptr,
+                         c->d_crow, c->d_centry, c->d_ebuf, c->d_rank, d_val, c->max_row);
+      MHD_HIP(hipGetLastError());
+    }
+  } else {
+    MHD_HIP(hipMemsetAsync(d_val, 0, sizeof(double) * c->nnz, s));
+  }
+  if (mode == 2 || c->ncells == 0) {
+  } else if (mode == 0 || c->color_ptr.size() < 2) {
+    hipLaunchKernelGGL((k_cell_jacobian<SCATTER_ATOMIC>), dim3(c->ncells), dim3(WG), 0, s, c->ncells,
+                       (const int32_t *)nullptr, c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, d_x, c->d_ref, c->d_rowptr,
+                       c->d_rank, d_val, c->phys);
+    MHD_HIP(hipGetLastError());
+  } else {
     for (size_t k = 0; k + 1 < c->color_ptr.size(); ++k) {
       const int n = c->color_ptr[k + 1] - c->color_ptr[k];
       if (n == 0) continue;
-      hipLaunchKernelGGL((k_cell_jacobian<false>), dim3(n), dim3(WG), 0, s, n, c->d_color_cells + c->color_ptr[k],
-                         c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, d_x, c->d_ref, c->d_rowptr, c->d_rank, d_val,
-                         c->phys);
+      hipLaunchKernelGGL((k_cell_jacobian<SCATTER_COLOUR>), dim3(n), dim3(WG), 0, s, n,
+                         c->d_color_cells + c->color_ptr[k], c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, d_x, c->d_ref,
+                         c->d_rowptr, c->d_rank, d_val, c->phys);
       MHD_HIP(hipGetLastError());
     }
   }

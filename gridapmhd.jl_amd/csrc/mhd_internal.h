@@ -99,10 +99,17 @@ struct mhd_ctx {
   std::vector<double> h_xyz;       // [ncells][24]
   int32_t nverts = 0;
   int32_t block_style = 0;
-  // cell colouring: no two cells of a class share a dof (deterministic assembly)
-  int32_t assembly_mode = 1;          // 0 atomic scatter, 1 colour classes (default)
+  // Jacobian scatter: 0 atomic, 1 colour classes, 2 element buffer + row assembly (default)
+  int32_t assembly_mode = 2;
+  bool dup_dofs = false;              // some cell holds a dof twice (forces mode 0)
+  // cell colouring: no two cells of a class share a dof
   std::vector<int32_t> color_ptr;     // [ncolors + 1]
   int32_t *d_color_cells = nullptr;   // cells sorted by colour
+  // row assembly: the (cell * NDOF + local row) pairs holding each free dof
+  int32_t *d_crow = nullptr;          // [n + 1]
+  int32_t *d_centry = nullptr;        // [crow[n]], increasing cell order per row
+  int32_t max_row = 0;                // longest CSR row
+  double *d_ebuf = nullptr;           // [ncells][NENT] element buffer (allocated on first use)
   // solver workspace (allocated lazily)
   struct Solver *solver = nullptr;
   // coarser level of the [u,j] block solve (owned), see mhd_attach_coarse
@@ -119,7 +126,11 @@ struct CoarseLink {
   double *d_Pval = nullptr, *d_Tval = nullptr;
   float *d_Pval32 = nullptr, *d_Tval32 = nullptr;
   int32_t *d_inj = nullptr;        // [nuc] fine u dof of every coarse u dof
-  int coarse_iters = 10;           // Schwarz-GMRES steps on the coarsest level
+  int coarse_iters = 10;           // Schwarz-GMRES steps on the coarsest level; 0: dense inverse
+  // coarsest level solved exactly: explicit inverse of its [u,j] block (column-major)
+  double *d_Ainv = nullptr;        // [n1c][n1c]
+  int32_t *d_piv = nullptr, *d_info = nullptr;
+  double *d_rhs = nullptr;         // [2 n1c] FP64 staging of FP32 vectors
   double omega = 0.2;              // damping of the Schwarz smoother inside the V-cycle
   void *rr = nullptr;              // [n1] fine residual (T)
   void *rc = nullptr;              // [n1c] coarse rhs (T)

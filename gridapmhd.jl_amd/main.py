@@ -78,8 +78,12 @@ def setup_fe_spaces(params) -> fespaces.Spaces:
 
 
 def _fe_operator(sp, params) -> _lib.Context:
-    """``_fe_operator`` (src/main.jl:207-233): the operator lives in libmhd."""
-    return _lib.context_from_spaces(sp, fluid_physics(params["fluid"]))
+    """``_fe_operator`` (src/main.jl:207-233): the operator lives in libmhd,
+    with the [u,j] V-cycle hierarchy of the solver parameters attached."""
+    s = params["solver"]
+    return _lib.context_with_levels(sp, fluid_physics(params["fluid"]), levels=s.get("levels", 1),
+                                    coarse_iters=int(s.get("coarse_iters", 0)),
+                                    smoother_weight=float(s.get("smoother_weight", 0.2)))
 
 
 def _solver_opts(ctx, params):

@@ -187,6 +187,18 @@ def _u_injection(sp_f, sp_c):
     return inj
 
 
+def auto_levels(mesh: CartesianMesh, min_cells: int = 136, min_xy: int = 8) -> int:
+    """Number of levels: halve x and y while both are even and >= `min_xy` and
+    the level still has more than `min_cells` cells (about 6,000 [u, j] dofs,
+    small enough for the exact coarsest solve)."""
+    nx, ny, nz = mesh.nc
+    levels = 1
+    while nx % 2 == 0 and ny % 2 == 0 and nx >= min_xy and ny >= min_xy and nx * ny * nz > min_cells:
+        nx, ny = nx // 2, ny // 2
+        levels += 1
+    return levels
+
+
 def hierarchy(sp: fespaces.Spaces, nlevels: int, u_tags, j_tags, thin_walls) -> List[fespaces.Spaces]:
     """[fine, coarse1, ...] spaces with the same boundary conditions."""
     out = [sp]

@@ -76,8 +76,11 @@ def default_solver_params(name: str) -> Dict[str, Any]:
             "rtol": 1e-6,
             "atol": 1e-8,
             "precision": "fp64",  # "fp64" | "mixed" | "mixed_bf16"
-            "restart": 100,
-            "inner_iters": 8,
+            "restart": 200,
+            "inner_iters": 3,     # flexible-GMRES steps of the inexact [u,j] solve
+            "levels": "auto",     # [u,j] V-cycle levels (x/y semi-coarsening); 1 = Schwarz only
+            "coarse_iters": 0,    # coarsest-level solve: 0 = exact (dense inverse), k = k Schwarz-GMRES steps
+            "smoother_weight": 0.2,
             "block_solvers": ["hip_vertex_schwarz", "mass_inverse", "mass_inverse"],
         }
     raise NotImplementedError(f"solver {name!r}: this backend provides 'hip' (and 'badia2024')")

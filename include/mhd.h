@@ -88,8 +88,8 @@ typedef struct {
   int32_t precision;            /* 0 FP64; 1 mixed: FP32 inner GMRES + FP32 smoother blocks inside FP64
                                    FGMRES refinement; 2 as 1 with BF16 smoother blocks */
   double rtol, atol;            /* Newton: ||R|| <= max(atol, rtol ||R0||) */
-  double ls_rtol;               /* linear: ||r|| <= ls_rtol ||b|| */
-  int32_t inner_iters;          /* smoothing sweeps of the [u,j] block solve */
+  double ls_rtol;               /* linear: ||r|| <= max(ls_rtol ||b||, max(atol, rtol ||R0||) / 2) */
+  int32_t inner_iters;          /* flexible-GMRES steps of the inexact [u,j] block solve */
   int32_t verbose;
 } mhd_solver_opts;
 
@@ -116,9 +116,12 @@ const char *mhd_last_error(const mhd_ctx *ctx);
 int mhd_set_params(mhd_ctx *ctx, const mhd_params *params);
 int mhd_set_dirichlet(mhd_ctx *ctx, const double *dir_vals);
 
-/* Jacobian scatter: 1 (default) = one launch per colour class of cells that
- * share no dof, plain read-modify-write, bitwise reproducible; 0 = one launch,
- * FP64 atomic adds (summation order not reproducible). */
+/* Jacobian scatter.  2 (default) = element matrices stored densely per cell,
+ * then every CSR row summed in on-chip memory and written once (coalesced,
+ * bitwise reproducible); 1 = one launch per colour class of cells that share
+ * no dof, plain read-modify-write (reproducible); 0 = one launch, FP64 atomic
+ * adds (summation order not reproducible).  Mode 2 holds ncells * 14913
+ * doubles of device memory from the first Jacobian on. */
 int mhd_set_assembly_mode(mhd_ctx *ctx, int32_t mode);
 
 /* Sizes of the system. */
@@ -192,8 +195,9 @@ int mhd_precond_setup(mhd_ctx *ctx, const mhd_solver_opts *opts);
 
 /* y += sum_P R_P^T A_P^-1 R_P r on the [u,j] block (device pointers, length
  * nfree[u] + nfree[j]; y is accumulated into).  The smoother of the [u,j]
- * block solve; needs mhd_precond_setup with precision 0. */
-int mhd_smoother_apply_d(mhd_ctx *ctx, const double *d_r, double *d_y, void *stream);
+ * block solve, in the precision of the last mhd_precond_setup: r and y are
+ * double for precision 0 and float (the inner-solve vectors) for 1 and 2. */
+int mhd_smoother_apply_d(mhd_ctx *ctx, const void *d_r, void *d_y, void *stream);
 /* Patch count, padded patch dimension, sum over patches of size^2. */
 int mhd_smoother_info(mhd_ctx *ctx, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries);
 

@@ -83,26 +83,28 @@ def test_manufactured_residual_vanishes():
         ctx.close()
 
 
-def test_colour_and_atomic_scatter_agree():
-    """Colour-class assembly (default) is bitwise reproducible and equals the
-    atomic scatter up to summation order."""
+@pytest.mark.parametrize("name", ["hunt_newton", "thin_wall_newton_al"])
+def test_scatter_modes_agree(name):
+    """Row assembly (default) and colour classes are bitwise reproducible; the
+    three scatter modes agree up to summation order."""
     import gridapmhd_amd as g
-    _say("[assembly] colour vs atomic, hunt 6x6x3")
-    sp, phys = cases.hunt_case(6, Ha=50.0)
+    _say(f"[assembly] scatter modes, {name}")
+    sp, phys = CASES[name]()
     rng = np.random.default_rng(7)
     x = 0.1 * rng.standard_normal(sp.ndofs)
     ctx = g._lib.context_from_spaces(sp, phys)
     try:
-        a1 = ctx.jacobian(x)
-        a2 = ctx.jacobian(x)
-        assert np.array_equal(a1, a2)
-        ctx.set_assembly_mode(0)
-        b = ctx.jacobian(x)
-        ctx.set_assembly_mode(1)
-        scale = np.max(np.abs(a1))
-        err = np.max(np.abs(a1 - b)) / scale
-        _say(f"  max diff / max entry {err:.3e}")
-        assert err < 1e-14
+        out = {}
+        for mode in (2, 1, 0):
+            ctx.set_assembly_mode(mode)
+            out[mode] = ctx.jacobian(x)
+            if mode != 0:
+                assert np.array_equal(out[mode], ctx.jacobian(x))
+        scale = np.max(np.abs(out[2]))
+        for mode in (1, 0):
+            err = np.max(np.abs(out[2] - out[mode])) / scale
+            _say(f"  mode {mode} vs 2: max diff / max entry {err:.3e}")
+            assert err < 1e-14
         with pytest.raises(g._lib.MhdError):
             ctx.set_assembly_mode(5)
     finally:
