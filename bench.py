@@ -127,6 +127,8 @@ def main():
     ap.add_argument("--levels", default="auto", help="levels of the [u,j] V-cycle (1 = Schwarz only; auto: 5 at C2)")
     ap.add_argument("--coarse-iters", type=int, default=0, help="0: exact coarsest solve")
     ap.add_argument("--weight", type=float, default=0.2, help="Schwarz smoother damping in the V-cycle")
+    ap.add_argument("--coarse-step", default="full", choices=["full", "minres"],
+                    help="coarse correction step (full: fewest iterations on this uniform duct)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -143,7 +145,7 @@ def main():
     t0 = time.perf_counter()
     levels = g.multigrid.auto_levels(sp.mesh) if args.levels == "auto" else int(args.levels)
     ctx = g._lib.context_with_levels(sp, phys, levels=levels, coarse_iters=args.coarse_iters,
-                                     smoother_weight=args.weight)
+                                     smoother_weight=args.weight, coarse_step=args.coarse_step)
     t_ctx = time.perf_counter() - t0
     log(f"context: nnz={ctx.nnz} ({t_ctx:.1f}s)")
     dev = torch.device("cuda:0")
@@ -200,7 +202,7 @@ def main():
     t0 = time.perf_counter()
     ref_levels = max(1, levels - 1)
     ctx_ref = g._lib.context_with_levels(sp, phys, levels=ref_levels, coarse_iters=args.coarse_iters,
-                                         smoother_weight=args.weight)
+                                         smoother_weight=args.weight, coarse_step="minres")
     o_ref = ctx_ref.default_opts(precision=0, rtol=3e-11, atol=1e-16, ls_rtol=1e-12, max_krylov=800, restart=400,
                                  inner_iters=args.inner, max_newton=2)
     x_ref, info_ref, rc_ref = ctx_ref.newton_solve(x0, o_ref)
@@ -292,6 +294,7 @@ def main():
                    "ndofs": sp.ndofs, "ncells": ncells, "nnz": nnz, "zeta": [1.0, 1.0],
                    "restart": args.restart, "inner_iters": args.inner, "levels": levels,
                    "coarse_iters": args.coarse_iters, "smoother_weight": args.weight,
+                   "coarse_step": args.coarse_step,
                    "newton_rtol": 1e-10, "linear_rtol": 1e-11},
         "fp64_newton_step_s": round(fp64["newton_step_s"], 4),
         "mixed_over_fp64_speedup": round(fp64["newton_step_s"] / mixed["newton_step_s"], 3),

@@ -75,7 +75,7 @@ EXPORTS = {
     "mhd_adjoint_solve": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(MhdSolverOpts),
                                          ctypes.POINTER(MhdSolveInfo)]),
     "mhd_attach_coarse": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
-                                         c_int32, c_double]),
+                                         c_int32, c_double, c_int32]),
     "mhd_smoother_apply_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhd_smoother_info": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
                                          ctypes.POINTER(c_int64)]),
@@ -109,6 +109,7 @@ def _ptr(a: Optional[np.ndarray]):
 
 
 CONVECTION = {"none": 0, "picard": 1, "newton": 2}
+COARSE_STEP = {"full": 0, "minres": 1}
 
 
 def make_params(phys: dict) -> MhdParams:
@@ -239,14 +240,18 @@ class Context:
             setattr(o, k, v)
         return o
 
-    def attach_coarse(self, coarse: "Context", P, inj_u, coarse_iters: int = 10, smoother_weight: float = 0.2):
-        """Hand `coarse` (and the prolongation P, scipy CSR) to this context."""
+    def attach_coarse(self, coarse: "Context", P, inj_u, coarse_iters: int = 0, smoother_weight: float = 0.2,
+                      coarse_step: str = "minres"):
+        """Hand `coarse` (and the prolongation P, scipy CSR) to this context.
+        coarse_step: "full" or "minres" (minimal-residual scaling of the
+        coarse correction)."""
         rowptr = np.ascontiguousarray(P.indptr, dtype=np.int64)
         col = np.ascontiguousarray(P.indices, dtype=np.int32)
         val = np.ascontiguousarray(P.data, dtype=np.float64)
         inj = np.ascontiguousarray(inj_u, dtype=np.int32)
         self._check(self.lib.mhd_attach_coarse(self.h, coarse.h, P.shape[0], P.shape[1], _ptr(rowptr), _ptr(col),
-                                               _ptr(val), _ptr(inj), int(coarse_iters), float(smoother_weight)),
+                                               _ptr(val), _ptr(inj), int(coarse_iters), float(smoother_weight),
+                                               COARSE_STEP[coarse_step]),
                     "mhd_attach_coarse")
         coarse.h = c_void_p()          # owned by this context now
         self._coarse = coarse
@@ -293,11 +298,13 @@ class Context:
         return x, info, rc
 
 
-def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.2, device=0) -> Context:
+def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.2, coarse_step="minres",
+                        device=0) -> Context:
     """Fine context plus `levels - 1` semi-coarsened levels attached for the
     [u,j] V-cycle (badia2024 numbering, nested x/y coarsening, the same
     boundary conditions on every level).  `levels="auto"` picks the depth
-    from the mesh; `coarse_iters = 0` solves the coarsest level exactly."""
+    from the mesh; `coarse_iters = 0` solves the coarsest level exactly;
+    `coarse_step` is "full" or "minres" (see mhd_attach_coarse)."""
     from . import fespaces, multigrid
     if levels == "auto":
         levels = multigrid.auto_levels(sp.mesh)
@@ -313,7 +320,7 @@ def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.2,
     # attach from the coarsest upwards so each attached context already carries its chain
     for lev in range(levels - 1, 0, -1):
         P, inj = multigrid.prolongation(chain[lev - 1], chain[lev])
-        ctxs[lev - 1].attach_coarse(ctxs[lev], P, inj, coarse_iters, smoother_weight)
+        ctxs[lev - 1].attach_coarse(ctxs[lev], P, inj, coarse_iters, smoother_weight, coarse_step)
     return ctx
 
 

@@ -132,7 +132,9 @@ struct CoarseLink {
   int32_t *d_piv = nullptr, *d_info = nullptr;
   double *d_rhs = nullptr;         // [2 n1c] FP64 staging of FP32 vectors
   double omega = 0.2;              // damping of the Schwarz smoother inside the V-cycle
-  void *rr = nullptr;              // [n1] fine residual (T)
+  int step_mode = 1;               // coarse correction step: 0 full, 1 minimal residual
+  void *rr = nullptr;              // [3 (n1 + 1)] fine residual, A11 yc, yc (T)
+  double *d_ab = nullptr;          // the two dots of the minimal-residual coarse step
   void *rc = nullptr;              // [n1c] coarse rhs (T)
   void *ec = nullptr;              // [n1c] coarse correction (T)
   int tprec = -1;                  // element type of rr/rc/ec: 0 double, 1 float

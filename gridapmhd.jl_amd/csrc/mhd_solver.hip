@@ -172,6 +172,18 @@ __global__ __launch_bounds__(256) void k_csr(int64_t nrows, const int64_t *__res
   if (row < nrows && lane == 0) y[row] = ACC ? (T)((double)y[row] + acc) : (T)acc;
 }
 
+// minimal-residual coarse correction: a = (rr . w) / (w . w) from dots[0..1],
+// y += a yc, rr -= a w  (w = A yc); a = 0 when w vanishes
+template <class T>
+__global__ void k_cc_update(int64_t n, const double *__restrict__ dots, const T *__restrict__ yc,
+                            const T *__restrict__ w, T *__restrict__ y, T *__restrict__ rr) {
+  const double a = dots[1] > 0.0 ? dots[0] / dots[1] : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    y[i] = (T)((double)y[i] + a * (double)yc[i]);
+    rr[i] = (T)((double)rr[i] - a * (double)w[i]);
+  }
+}
+
 // dense column-major copy of the [u,j] block (the A11 prefix of each row)
 __global__ void k_dense_a11(int64_t n1, const int64_t *__restrict__ rowptr, const int64_t *__restrict__ split,
                             const int32_t *__restrict__ colind, const double *__restrict__ val,
@@ -692,6 +704,17 @@ int coarse_direct(mhd_ctx *c, const T *rc, T *ec, int prec) {
   return 0;
 }
 
+// One V-cycle on the [u,j] block of level c: y = M^-1 r with damped
+// vertex-star Schwarz pre- and post-smoothing around the coarse correction.
+// The coarse correction yc = P A11c^-1 P^T (r - A11 y) enters with the full
+// step (coarse_step 0) or with a = argmin ||r - A11 (y + a yc)|| (coarse_step
+// 1): the coarse levels are rediscretised, not Galerkin products, and where
+// they misrepresent the fine operator (thin boundary layers under-resolved at
+// high Ha on coarse meshes) the full step can ruin the cycle while the
+// minimal-residual step only rescales it.  Where the coarse levels are good
+// the full step converges in fewer iterations.  The coarsest
+// level is solved exactly (dense inverse) or by coarse_iters steps of
+// Schwarz-preconditioned FGMRES.
 template <class T>
 int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
   Solver *S = c->solver;
@@ -702,17 +725,18 @@ int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
   mhd_ctx *cc = L->ctx;
   Solver *SC = cc->solver;
   constexpr int LPR = 16;
-  T *rr = (T *)L->rr, *rcv = (T *)L->rc, *ec = (T *)L->ec;
+  const int64_t ld = L->n1 + 1;
+  T *rr = (T *)L->rr, *wv = rr + ld, *yc = rr + 2 * ld;
+  T *rcv = (T *)L->rc, *ec = (T *)L->ec;
   const int gfine = (int)((S->n1 * LPR + 255) / 256), gc = (int)((L->n1c * LPR + 255) / 256);
-  auto resid = [&]() {
-    if (prec == 0)
-      hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1,
-                         c->d_rowptr, S->d_split, c->d_colind, c->d_val, y, r, rr);
-    else
-      hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1,
-                         c->d_rowptr, S->d_split, c->d_colind, S->d_val32, y, r, rr);
-  };
-  resid();
+  // rr = r - A11 y
+  if (prec == 0)
+    hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
+                       S->d_split, c->d_colind, c->d_val, y, r, rr);
+  else
+    hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
+                       S->d_split, c->d_colind, S->d_val32, y, r, rr);
+  // restriction
   if (prec == 0)
     hipLaunchKernelGGL((k_csr<double, T, LPR, false>), dim3(gc), dim3(256), 0, c->stream, L->n1c, L->d_Trow, L->d_Tcol,
                        L->d_Tval, rr, rcv);
@@ -729,17 +753,40 @@ int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
     if ((rc = inner_solve<T>(cc, prec))) return rc;
     MHD_HIP(hipMemcpyAsync(ec, SC->x1, sizeof(T) * L->n1c, hipMemcpyDeviceToDevice, c->stream));
   }
-  if (prec == 0)
-    hipLaunchKernelGGL((k_csr<double, T, LPR, true>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow, L->d_Pcol,
-                       L->d_Pval, ec, y);
-  else
-    hipLaunchKernelGGL((k_csr<float, T, LPR, true>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow, L->d_Pcol,
-                       L->d_Pval32, ec, y);
-  resid();
+  if (L->step_mode == 0) {  // y += P ec, rr = r - A11 y
+    if (prec == 0) {
+      hipLaunchKernelGGL((k_csr<double, T, LPR, true>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow,
+                         L->d_Pcol, L->d_Pval, ec, y);
+      hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1,
+                         c->d_rowptr, S->d_split, c->d_colind, c->d_val, y, r, rr);
+    } else {
+      hipLaunchKernelGGL((k_csr<float, T, LPR, true>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow,
+                         L->d_Pcol, L->d_Pval32, ec, y);
+      hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1,
+                         c->d_rowptr, S->d_split, c->d_colind, S->d_val32, y, r, rr);
+    }
+    MHD_HIP(hipGetLastError());
+    return patch_as<T>(c, rr, y, prec, L->omega, true);
+  }
+  // yc = P ec, w = A11 yc
+  if (prec == 0) {
+    hipLaunchKernelGGL((k_csr<double, T, LPR, false>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow,
+                       L->d_Pcol, L->d_Pval, ec, yc);
+    hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 1>), dim3(gfine), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
+                       S->d_split, c->d_colind, c->d_val, yc, (const T *)nullptr, wv);
+  } else {
+    hipLaunchKernelGGL((k_csr<float, T, LPR, false>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow,
+                       L->d_Pcol, L->d_Pval32, ec, yc);
+    hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 1>), dim3(gfine), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
+                       S->d_split, c->d_colind, S->d_val32, yc, (const T *)nullptr, wv);
+  }
+  MHD_HIP(hipGetLastError());
+  // (rr . w, w . w) on the device; y += a yc, rr -= a w
+  if ((rc = mdot(c, rr, ld, 2, wv, S->n1, L->d_ab))) return rc;
+  hipLaunchKernelGGL((k_cc_update<T>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, L->d_ab, yc, wv, y, rr);
   MHD_HIP(hipGetLastError());
   // post-smoothing accumulates into y
   if ((rc = patch_as<T>(c, rr, y, prec, L->omega, true))) return rc;
-  MHD_HIP(hipGetLastError());
   return 0;
 }
 
@@ -856,7 +903,8 @@ int coarse_alloc(mhd_ctx *c, int prec) {
   for (void *p : ptrs)
     if (p) MHD_HIP(hipFree(p));
   const size_t es = tp == 0 ? sizeof(double) : sizeof(float);
-  MHD_HIP(hipMalloc(&L->rr, es * (L->n1 + 1)));
+  MHD_HIP(hipMalloc(&L->rr, es * 3 * (L->n1 + 1)));  // rr, w, yc (stride n1 + 1)
+  if (!L->d_ab) MHD_HIP(hipMalloc(&L->d_ab, sizeof(double) * 2));
   MHD_HIP(hipMalloc(&L->rc, es * (L->n1c + 1)));
   MHD_HIP(hipMalloc(&L->ec, es * (L->n1c + 1)));
   L->tprec = tp;
@@ -1046,7 +1094,7 @@ void mhd_coarse_free(mhd_ctx *c) {
   if (!c || !c->coarse) return;
   CoarseLink *L = c->coarse;
   void *ptrs[] = {L->d_Prow, L->d_Trow, L->d_Pcol, L->d_Tcol, L->d_Pval, L->d_Tval, L->d_Pval32, L->d_Tval32,
-                  L->d_inj, L->rr, L->rc, L->ec, L->d_Ainv, L->d_piv, L->d_info, L->d_rhs};
+                  L->d_inj, L->rr, L->rc, L->ec, L->d_Ainv, L->d_piv, L->d_info, L->d_rhs, L->d_ab};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (L->ctx) mhd_destroy(L->ctx);
@@ -1058,7 +1106,11 @@ extern "C" {
 
 int mhd_attach_coarse(mhd_ctx *c, mhd_ctx *coarse, int64_t nrows, int64_t ncols, const int64_t *P_rowptr,
                       const int32_t *P_col, const double *P_val, const int32_t *inj_u, int32_t coarse_iters,
-                      double smoother_weight) {
+                      double smoother_weight, int32_t coarse_step) {
+  if (coarse_step != 0 && coarse_step != 1) {
+    if (c) c->err = "mhd_attach_coarse: coarse_step must be 0 (full) or 1 (minimal residual)";
+    return -1;
+  }
   if (!(smoother_weight > 0.0 && smoother_weight <= 1.0)) {
     if (c) c->err = "mhd_attach_coarse: smoother_weight must be in (0, 1]";
     return -1;
@@ -1097,6 +1149,7 @@ int mhd_attach_coarse(mhd_ctx *c, mhd_ctx *coarse, int64_t nrows, int64_t ncols,
     return -1;
   }
   L->omega = smoother_weight;
+  L->step_mode = coarse_step;
   // transpose on the host
   std::vector<int64_t> trow(ncols + 1, 0);
   for (int64_t k = 0; k < nnzP; ++k) trow[P_col[k] + 1]++;

@@ -69,7 +69,11 @@ function GridapMHD.default_solver_params(::Val{:hip})
     :solver_postpro => ((cache, info) -> hip_postpro(cache, info)),
     :niter => 20, :niter_ls => 400, :rtol => 1e-6, :atol => 1e-8,
     :precision => :fp64,           # :fp64 | :mixed | :mixed_bf16
-    :inner_iters => 8, :restart => 100,
+    :inner_iters => 3, :restart => 200,
+    :levels => :auto,              # [u,j] V-cycle levels (x/y semi-coarsening), see attach_levels!
+    :coarse_iters => 0,            # 0: exact coarsest solve
+    :smoother_weight => 0.2,
+    :coarse_step => :minres,       # :minres | :full
   )
 end
 GridapMHD.uses_petsc(::Val{:hip}) = false
@@ -113,20 +117,21 @@ function Gridap.Algebra.solve!(x::AbstractVector, ns::HIPNewtonSolver, op::Grida
 end
 
 """
-    attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=10, ω=0.2)
+    attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=0, ω=0.2, step=:minres)
 
 Chains coarser contexts (built from `hunt_generate_mesh_hierarchy`-style
 x,y-semi-coarsened models) onto `ctx` through `mhd_attach_coarse`; each
 prolongation is a `SparseMatrixCSR` fine x coarse on the [u; j] dofs.
 """
-function attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=10, ω=0.2)
+function attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=0, ω=0.2, step=:minres)
   for lev in length(coarse_ctxs):-1:1
     fine = lev == 1 ? ctx : coarse_ctxs[lev-1]
     P = prolongations[lev]
     rc = ccall((:mhd_attach_coarse, libmhd), Cint,
-               (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Ptr{Int64}, Ptr{Int32}, Ptr{Float64}, Ptr{Int32}, Int32, Float64),
+               (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int64, Ptr{Int64}, Ptr{Int32}, Ptr{Float64}, Ptr{Int32}, Int32, Float64,
+                Int32),
                fine, coarse_ctxs[lev], size(P, 1), size(P, 2), P.rowptr .- 1, Int32.(P.colval .- 1), P.nzval,
-               Int32.(injections[lev] .- 1), coarse_iters, ω)
+               Int32.(injections[lev] .- 1), coarse_iters, ω, step == :full ? 0 : 1)
     check(rc, fine, "mhd_attach_coarse")
   end
 end

@@ -81,6 +81,7 @@ def default_solver_params(name: str) -> Dict[str, Any]:
             "levels": "auto",     # [u,j] V-cycle levels (x/y semi-coarsening); 1 = Schwarz only
             "coarse_iters": 0,    # coarsest-level solve: 0 = exact (dense inverse), k = k Schwarz-GMRES steps
             "smoother_weight": 0.2,
+            "coarse_step": "minres",  # coarse correction step: "minres" (robust) | "full"
             "block_solvers": ["hip_vertex_schwarz", "mass_inverse", "mass_inverse"],
         }
     raise NotImplementedError(f"solver {name!r}: this backend provides 'hip' (and 'badia2024')")

@@ -181,11 +181,16 @@ int mhd_adjoint_solve(mhd_ctx *ctx, const double *x_state, const double *rhs, do
  * ownership of coarse (destroyed with ctx; do not destroy it separately).  The
  * [u,j] smoother of mhd_newton_solve then becomes a V-cycle whose vertex-star
  * Schwarz pre/post-smoothing is damped by smoother_weight (0, 1] (the patches
- * overlap up to 8-fold; 0.2 is a good value); on the coarsest level
- * coarse_iters Schwarz-GMRES steps are taken. */
+ * overlap up to 8-fold; 0.2 is a good value).  On the coarsest level
+ * coarse_iters Schwarz-GMRES steps are taken, or, with coarse_iters = 0, the
+ * level is solved exactly with a dense FP64 inverse (at most 32768 [u,j]
+ * dofs there).  coarse_step 0 adds the coarse correction with a full step;
+ * 1 scales it by the minimal-residual step (robust where the rediscretised
+ * coarse level misrepresents the fine one, e.g. high Ha on coarse meshes). */
 int mhd_attach_coarse(mhd_ctx *ctx, mhd_ctx *coarse, int64_t nrows, int64_t ncols, const int64_t *P_rowptr,
                       const int32_t *P_col, const double *P_val, const int32_t *inj_u, int32_t coarse_iters,
-                      double smoother_weight);
+                      double smoother_weight,
+                      int32_t coarse_step);
 
 /* Preconditioner set-up for the last assembled Jacobian: the numerical_setup!
  * of the block solver (src/Solvers/badia2024.jl:20-31) -- Schur scalings,

@@ -45,9 +45,13 @@ def hunt_case(cx, Ha=50.0, nz=3, BL_adapted=False, domain=(-1.0, 1.0, -1.0, 1.0,
 
 
 def duct_thin_wall_case(nc, Ha, Re, cw, tau=100.0, domain=(-1.0, 1.0, -1.0, 1.0, 0.0, 0.3),
-                        convection="newton", map=None, zeta_u=0.0, zeta_j=0.0, style="consecutive"):
-    """Duct with thin conducting walls on all four walls (no j Dirichlet)."""
+                        convection="newton", map=None, zeta_u=0.0, zeta_j=0.0, style="consecutive", jitter=0.0):
+    """Duct with thin conducting walls on all four walls (no j Dirichlet);
+    `jitter` moves interior nodes by U[-jitter h, jitter h]^3 (seed 20261202),
+    which makes the cell maps non-affine."""
     mesh = g.meshers.hunt_generate_base_mesh(nc[:2], nz=nc[2], domain=domain, Ha=Ha, map=map or (lambda x: x))
+    if jitter:
+        g.meshers.jitter_nodes(mesh, jitter, 20261202)
     sp = g.fespaces.setup_fe_spaces(mesh, "noslip", [], [dict(domain="walls", cw=cw, tau=tau, jw=0.0)], style=style)
     return sp, cfd_physics(Ha, Re, convection=convection, zeta_u=zeta_u, zeta_j=zeta_j)
 

@@ -38,6 +38,9 @@ CASES = {
     "thin_wall_picard_tanh": lambda: cases.duct_thin_wall_case((6, 6, 3), Ha=30.0, Re=5.0, cw=0.1,
                                                                convection="picard",
                                                                map=__import__("gridapmhd_amd").meshers.tanh_map(3.0)),
+    # non-affine cells: interior nodes jittered by +-0.2 h (the combined configuration's mesh)
+    "jitter_newton_al": lambda: cases.duct_thin_wall_case((5, 5, 3), Ha=500.0, Re=50.0, cw=0.001,
+                                                          convection="newton", zeta_u=1.0, zeta_j=1.0, jitter=0.2),
 }
 
 
