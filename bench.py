@@ -36,6 +36,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Newton-step s (asm+solve, 1.5M DOF) mixed vs FP64 at 1e-8 rel err; cells/s asm"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+# HBM bytes per launch of the hot kernels at 64 x 64 x 8, measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) over
+# `bench.py --kernels-only`, each scaled by what the counter reads for a 1 GiB
+# device-to-device copy (FETCH_SIZE x 2, WRITE_SIZE x 1).
+C2_TRAFFIC = {"smoother_f32": 6.739e9, "smoother_f64": 1.3924e10, "spmv_f64": 5.852e9, "jacobian": 1.3635e10}
+# k_cell_jacobian at 64 x 64 x 8: SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 flops and
+# SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE per XCD x 1024 SIMDs), same runs.
+C2_JACOBIAN_MFMA = {"executed_flops": 5.979e10, "mfma_busy_frac": 0.176, "cell_kernel_ms": 4.36}
 
 
 def log(msg):
@@ -114,6 +124,37 @@ def cpu_baseline(cfg, sp, phys, budget_s=12.0):
                        f"{dt2:.1f} s on {cores} threads ({model})")
 
 
+def kernels_only(ctx, sp, x, val, r, stream, sh, times):
+    """A 1 GiB device-to-device copy (reads and writes 2^30 bytes: the
+    calibration of FETCH_SIZE / WRITE_SIZE), then 3 launches each of the
+    Jacobian, the SpMV and the fine-level smoother in FP64 and FP32."""
+    import torch
+    dev = x.device
+    a = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t_copy = time_events(lambda: b.copy_(a), 5, stream)
+    del a, b
+    for _ in range(3):
+        ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh)
+        ctx.spmv_d(x.data_ptr(), r.data_ptr(), sh)
+    n1 = sp.nfree["u"] + sp.nfree["j"]
+    out = {k: round(v * 1e3, 4) for k, v in times.items()}
+    out["copy_1GiB"] = round(t_copy * 1e3, 4)
+    for prec, dt in ((0, torch.float64), (1, torch.float32)):
+        ctx.precond_setup(ctx.default_opts(precision=prec))
+        rr = torch.rand(n1, dtype=dt, device=dev)
+        yy = torch.zeros(n1, dtype=dt, device=dev)
+        for _ in range(3):
+            ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh)
+        out[f"smoother_prec{prec}"] = round(time_events(lambda: ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh),
+                                                        5, stream) * 1e3, 4)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernels_ms": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +171,9 @@ def main():
     ap.add_argument("--coarse-step", default="full", choices=["full", "minres"],
                     help="coarse correction step (full: fewest iterations on this uniform duct)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time the hot kernels only (a 1 GiB device copy for counter calibration, Jacobian, SpMV, "
+                         "fine-level smoother in FP64 and FP32), no solves: the run the PMC passes profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
@@ -144,6 +188,8 @@ def main():
     log(f"mesh+spaces {cfg['nc']}: ndofs={sp.ndofs} ({time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
     levels = g.multigrid.auto_levels(sp.mesh) if args.levels == "auto" else int(args.levels)
+    if args.kernels_only:
+        levels = 1
     ctx = g._lib.context_with_levels(sp, phys, levels=levels, coarse_iters=args.coarse_iters,
                                      smoother_weight=args.weight, coarse_step=args.coarse_step)
     t_ctx = time.perf_counter() - t0
@@ -166,9 +212,9 @@ def main():
         ctx.residual_d(x.data_ptr(), r.data_ptr(), sh)
     torch.cuda.synchronize()
     t_jac = time_events(lambda: ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh), args.asm_reps, stream)
-    val_ref = val.clone()
-    t_modes, diff_modes = {}, {}
-    for mode in (1, 0):     # colour classes, atomic scatter
+    t_modes, diff_modes = {0: float("nan"), 1: float("nan")}, {0: float("nan"), 1: float("nan")}
+    val_ref = None if args.kernels_only else val.clone()
+    for mode in () if args.kernels_only else (1, 0):     # colour classes, atomic scatter
         ctx.set_assembly_mode(mode)
         ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh)
         t_modes[mode] = time_events(lambda: ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh), args.asm_reps, stream)
@@ -186,6 +232,11 @@ def main():
     t_spmv = time_events(lambda: ctx.spmv_d(x.data_ptr(), y.data_ptr(), sh), 20, stream)
     spmv_bytes = nnz * (8 + 4) + (sp.ndofs + 1) * 8 + 2 * sp.ndofs * 8
     log(f"spmv {t_spmv * 1e3:.3f} ms ({spmv_bytes / t_spmv / 1e9:.0f} GB/s)")
+
+    if args.kernels_only:
+        kernels_only(ctx, sp, x, val, r, stream, sh, dict(jacobian=t_jac, residual=t_res, spmv=t_spmv))
+        ctx.close()
+        return
 
     # ---- solves ------------------------------------------------------------
     x0 = np.zeros(sp.ndofs)
@@ -269,11 +320,24 @@ def main():
         cpu = cpu_baseline(cfg, sp, phys)
         log(f"cpu: {cpu}")
 
-    def roof(name, nbytes, t):
+    c2 = args.config == "c2"
+
+    def roof(name, nbytes, t, traffic_key=None):
         a = nbytes / t / 1e9
+        traffic = int(C2_TRAFFIC[traffic_key]) if (c2 and traffic_key) else None
         return dict(kernel=name, bound="hbm", achieved=round(a, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(a / HBM_PEAK_GBS, 4), traffic=None, bytes_per_launch=int(nbytes),
+                    frac=round(a / HBM_PEAK_GBS, 4), traffic=traffic, bytes_per_launch=int(nbytes),
                     avg_launch_ms=round(t * 1e3, 4))
+
+    jac_roof = roof("jacobian assembly (k_cell_jacobian + k_row_assemble + k_face_jacobian)", jac_bytes, t_jac,
+                    "jacobian")
+    if c2:
+        m = C2_JACOBIAN_MFMA
+        jac_roof["mfma_f64"] = dict(executed_flops_per_launch=m["executed_flops"],
+                                    algorithmic_flops_per_launch=round(1.34e6 * ncells),
+                                    achieved_tflops=round(m["executed_flops"] / (m["cell_kernel_ms"] * 1e-3) / 1e12, 2),
+                                    peak_tflops=FP64_PEAK_TFS, mfma_busy_frac=m["mfma_busy_frac"],
+                                    kernel="k_cell_jacobian")
 
     mixed, fp64 = results["mixed"], results["fp64"]
     out = {
@@ -308,11 +372,12 @@ def main():
         "residual_cells_per_s": round(ncells / t_res, 1),
         # the dominant kernel of the mixed Newton step: the fine-level smoother
         "roofline": roof(f"k_patch_apply<{'float' if args.mixed == 1 else 'bf16'}> vertex-star Schwarz, "
-                         f"fine level ({npatch} patches)", sm_bytes[args.mixed], t_sm[args.mixed]),
-        "roofline_kernels": [roof("k_patch_apply<double> vertex-star Schwarz, fine level", sm_bytes[0], t_sm[0]),
-                             roof("k_spmv fp64 CSR", spmv_bytes, t_spmv),
-                             roof("jacobian assembly (k_cell_jacobian + k_row_assemble + k_face_jacobian)",
-                                  jac_bytes, t_jac)],
+                         f"fine level ({npatch} patches)", sm_bytes[args.mixed], t_sm[args.mixed],
+                         "smoother_f32" if args.mixed == 1 else None),
+        "roofline_kernels": [roof("k_patch_apply<double> vertex-star Schwarz, fine level", sm_bytes[0], t_sm[0],
+                                  "smoother_f64"),
+                             roof("k_spmv fp64 CSR", spmv_bytes, t_spmv, "spmv_f64"),
+                             jac_roof],
         "reference_solve": dict(seconds=round(t_ref, 2), levels=ref_levels, newton=info_ref.newton_iters,
                                 krylov=info_ref.krylov_iters, rel_residual=info_ref.residual / info_ref.residual0),
         "context_setup_s": round(t_ctx, 2),
