@@ -124,6 +124,84 @@ def cpu_baseline(cfg, sp, phys, budget_s=12.0):
                        f"{dt2:.1f} s on {cores} threads ({model})")
 
 
+def adjoint_config(args):
+    """Differentiation configuration: developing duct 64 x 64 x 8 (inlet
+    profile at z = 0, outlet at z = 0.3, thin walls), Ha = 100, Re = 10,
+    cw = 0.01; J = pressure drop.  Tangent (forward-sensitivity) and adjoint
+    gradients against central differences of warm-started Newton solves at
+    relative steps 1e-4 and 1e-6.  The FD error floor is the FP64 residual
+    floor divided by the step: about 1e-7 at 1e-4, 1e-5 at 1e-6."""
+    import torch
+    import gridapmhd_amd as g
+    from gridapmhd_amd import adjoint as adj
+    nc, Ha, Re, cw = (64, 64, 8), 100.0, 10.0, 0.01
+    t0 = time.perf_counter()
+    sp, phys = adj.inlet_duct(nc, Ha, Re, cw)
+    ctx = g._lib.context_with_levels(sp, phys, levels="auto", coarse_iters=0, coarse_step="minres")
+    log(f"developing duct {nc}: ndofs={sp.ndofs} ({time.perf_counter() - t0:.1f}s)")
+
+    def opts(c, rtol, atol, ls_rtol=1e-6):
+        return c.default_opts(precision=0, rtol=rtol, atol=atol, ls_rtol=ls_rtol, max_krylov=800, restart=200,
+                              inner_iters=3, max_newton=12)
+    t0 = time.perf_counter()
+    x, info, rc = ctx.newton_solve(np.zeros(sp.ndofs), opts(ctx, 3e-11, 1e-30))
+    t_state = time.perf_counter() - t0
+    r0 = info.residual0
+    log(f"state: newton={info.newton_iters} krylov={info.krylov_iters} relres={info.residual / r0:.2e} rc={rc} "
+        f"({t_state:.1f}s)")
+    oa = opts(ctx, 1e-10, 1e-30, ls_rtol=1e-9)
+    t0 = time.perf_counter()
+    tinfo = {}
+    J, tHa, tcw = adj.tangent_gradient(ctx, sp, x, Ha, Re, oa, info_out=tinfo)
+    t_tan = time.perf_counter() - t0
+    log(f"tangent: J={J:.12g} dJ/dHa={tHa:.10g} dJ/dcw={tcw:.10g} {tinfo} ({t_tan:.1f}s)")
+    t0 = time.perf_counter()
+    ainfo = {}
+    try:
+        _, dHa, dcw = adj.gradient(ctx, sp, x, Ha, Re, oa, info_out=ainfo)
+    except RuntimeError:
+        dHa = dcw = float("nan")
+    t_adj = time.perf_counter() - t0
+    log(f"adjoint: dJ/dHa={dHa:.10g} dJ/dcw={dcw:.10g} {ainfo} ({t_adj:.1f}s)")
+    gvec = adj.pressure_drop_functional(sp)
+    floor = 2e-11 * r0
+
+    def J_Ha(Ha_):
+        ctx.set_params(dict(phys, gamma=Ha_ * Ha_ / Re))
+        xs, inf, _ = ctx.newton_solve(x, opts(ctx, 1e-9, floor))
+        ctx.set_params(phys)
+        return float(gvec @ xs)
+
+    def J_cw(cw_):
+        sp2, phys2 = adj.inlet_duct(nc, Ha, Re, cw_)
+        c2 = g._lib.context_with_levels(sp2, phys2, levels="auto", coarse_iters=0, coarse_step="minres")
+        try:
+            xs, inf, _ = c2.newton_solve(x, opts(c2, 1e-9, floor))
+        finally:
+            c2.close()
+        return float(gvec @ xs)
+    fd = {}
+    t0 = time.perf_counter()
+    for h in (1e-4, 1e-6):
+        fHa = (J_Ha(Ha * (1 + h)) - J_Ha(Ha * (1 - h))) / (2 * h * Ha)
+        fcw = (J_cw(cw * (1 + h)) - J_cw(cw * (1 - h))) / (2 * h * cw)
+        fd[h] = dict(dJ_dHa=fHa, dJ_dcw=fcw, tangent_rel_diff_Ha=abs(tHa / fHa - 1),
+                     tangent_rel_diff_cw=abs(tcw / fcw - 1), adjoint_rel_diff_Ha=abs(dHa / fHa - 1),
+                     adjoint_rel_diff_cw=abs(dcw / fcw - 1))
+        log(f"FD h={h:g}: dJ/dHa={fHa:.10g} dJ/dcw={fcw:.10g} {fd[h]}")
+    t_fd = time.perf_counter() - t0
+    ctx.close()
+    print(json.dumps({"adjoint": {"workload": f"developing thin-wall duct {'x'.join(map(str, nc))}, Ha={Ha:g}, "
+                                              f"Re={Re:g}, cw={cw:g}, J = pressure drop z=0 -> z=0.3",
+                                  "ndofs": sp.ndofs, "J": J, "tangent": {"dJ_dHa": tHa, "dJ_dcw": tcw,
+                                                                         "seconds": round(t_tan, 2), "solves": tinfo},
+                                  "adjoint": {"dJ_dHa": dHa, "dJ_dcw": dcw, "seconds": round(t_adj, 2),
+                                              "solve": ainfo},
+                                  "state_solve_s": round(t_state, 2),
+                                  "fd_s": round(t_fd, 2), "fd": {f"{h:g}": v for h, v in fd.items()}}}),
+          flush=True)
+
+
 def kernels_only(ctx, sp, x, val, r, stream, sh, times):
     """A 1 GiB device-to-device copy (reads and writes 2^30 bytes: the
     calibration of FETCH_SIZE / WRITE_SIZE), then 3 launches each of the
@@ -171,6 +249,9 @@ def main():
     ap.add_argument("--coarse-step", default="full", choices=["full", "minres"],
                     help="coarse correction step (full: fewest iterations on this uniform duct)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="the differentiation configuration instead: pressure-drop gradient w.r.t. (Ha, cw) of the "
+                         "developing 64x64x8 duct by the adjoint, against central differences")
     ap.add_argument("--kernels-only", action="store_true",
                     help="time the hot kernels only (a 1 GiB device copy for counter calibration, Jacobian, SpMV, "
                          "fine-level smoother in FP64 and FP32), no solves: the run the PMC passes profile")
@@ -182,6 +263,9 @@ def main():
     import torch
     import gridapmhd_amd as g
 
+    if args.adjoint:
+        adjoint_config(args)
+        return
     cfg = CONFIGS[args.config]
     t0 = time.perf_counter()
     sp, phys = build_problem(cfg)

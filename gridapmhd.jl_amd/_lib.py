@@ -74,6 +74,8 @@ EXPORTS = {
     "mhd_dresidual_dparam": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_void_p]),
     "mhd_adjoint_solve": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(MhdSolverOpts),
                                          ctypes.POINTER(MhdSolveInfo)]),
+    "mhd_linear_solve": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(MhdSolverOpts),
+                                        ctypes.POINTER(MhdSolveInfo)]),
     "mhd_attach_coarse": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                          c_int32, c_double, c_int32]),
     "mhd_smoother_apply_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -262,6 +264,17 @@ class Context:
         out = np.empty(self.n)
         self._check(self.lib.mhd_dresidual_dparam(self.h, _ptr(x), int(param), _ptr(out)), "mhd_dresidual_dparam")
         return out
+
+    def linear_solve(self, x, rhs, opts: Optional[MhdSolverOpts] = None):
+        """s = A(x)^-1 rhs on the GPU (forward preconditioner); returns (s, info, rc)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        sol = np.empty(self.n)
+        opts = opts or self.default_opts()
+        info = MhdSolveInfo()
+        rc = self._check(self.lib.mhd_linear_solve(self.h, _ptr(x), _ptr(rhs), _ptr(sol), ctypes.byref(opts),
+                                                   ctypes.byref(info)), "mhd_linear_solve")
+        return sol, info, rc
 
     def adjoint_solve(self, x, rhs, opts: Optional[MhdSolverOpts] = None):
         """lambda = A(x)^-T rhs on the GPU; returns (lambda, info, rc)."""

@@ -954,8 +954,18 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restri
   const int lane = threadIdx.x % LPR;
   double acc = 0.0;
   if (row < n) {
-    const int64_t b = rowptr[row], e = rowptr[row + 1];
-    for (int64_t p = b + lane; p < e; p += LPR) acc += val[p] * x[colind[p]];
+    // four independent accumulators: four loads in flight per lane
+    const int64_t e = rowptr[row + 1];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t p = rowptr[row] + lane;
+    for (; p + 3 * LPR < e; p += 4 * LPR) {
+      a0 += val[p] * x[colind[p]];
+      a1 += val[p + LPR] * x[colind[p + LPR]];
+      a2 += val[p + 2 * LPR] * x[colind[p + 2 * LPR]];
+      a3 += val[p + 3 * LPR] * x[colind[p + 3 * LPR]];
+    }
+    for (; p < e; p += LPR) a0 += val[p] * x[colind[p]];
+    acc = (a0 + a1) + (a2 + a3);
   }
   for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, LPR);
   if (row < n && lane == 0) y[row] = acc;

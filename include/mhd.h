@@ -165,11 +165,17 @@ void mhd_default_solver_opts(mhd_solver_opts *opts);
  * Lorentz term -int (j x B).v) or cw (param 1: int_G tau (s.n) n.(grad j).n
  * over every thin-wall face).  dR/dHa = (2 Ha / Re) dR/dgamma in the :cfd scaling.
  * mhd_adjoint_solve: lambda = A(x)^-T rhs with the transposed Badia2024
- * preconditioner (lower block-triangular), single level.  Then
- * dJ/dtheta = dJ/dtheta|_explicit - lambda . dR/dtheta. */
+ * preconditioner (lower block-triangular; an attached V-cycle hierarchy works
+ * on the transposed Jacobian of every level).  Then
+ * dJ/dtheta = dJ/dtheta|_explicit - lambda . dR/dtheta.
+ * mhd_linear_solve: sol = A(x)^-1 rhs with the forward preconditioner; the
+ * tangent (forward-sensitivity) form of the same gradient,
+ * dJ/dtheta = dJ/dx . A^-1 (-dR/dtheta), one solve per parameter. */
 int mhd_dresidual_dparam(mhd_ctx *ctx, const double *x_free, int32_t param, double *out);
 int mhd_adjoint_solve(mhd_ctx *ctx, const double *x_state, const double *rhs, double *lambda,
                       const mhd_solver_opts *opts, mhd_solve_info *info);
+int mhd_linear_solve(mhd_ctx *ctx, const double *x_state, const double *rhs, double *sol,
+                     const mhd_solver_opts *opts, mhd_solve_info *info);
 
 /* Attach a coarser level to the [u,j]-block solve (geometric coarse-grid
  * correction; the x,y semi-coarsened hierarchy of the reference's GMG,
