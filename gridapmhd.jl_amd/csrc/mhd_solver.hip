@@ -201,6 +201,26 @@ __global__ void k_cc_update(int64_t n, const double *__restrict__ dots, const T 
   }
 }
 
+// Adjoint scaling D^-1 A^T D with D = 1 on u, p and s on j, phi (block
+// numbering [u; j], p, phi): A^T carries A's Lorentz couplings swapped, sigma X
+// in the u rows and -gamma X^T in the j rows; with s = gamma / sigma the scaled
+// operator has A's coupling magnitudes again, which the Schur and Krylov parts
+// of the preconditioner are tuned to (row weights of the 2-norm residual).
+__device__ inline double adj_d(int64_t i, int64_t nu, int64_t n1, int64_t np, double s) {
+  return ((i >= nu && i < n1) || i >= n1 + np) ? s : 1.0;
+}
+__global__ void k_adj_scale(int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
+                            double *__restrict__ val, int64_t nu, int64_t n1, int64_t np, double s) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const double ir = 1.0 / adj_d(r, nu, n1, np, s);
+    for (int64_t p = rowptr[r]; p < rowptr[r + 1]; ++p) val[p] *= adj_d(colind[p], nu, n1, np, s) * ir;
+  }
+}
+__global__ void k_adj_scale_vec(int64_t n, int64_t nu, int64_t n1, int64_t np, double f, double *__restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if ((i >= nu && i < n1) || i >= n1 + np) x[i] *= f;
+}
+
 // dense column-major copy of the [u,j] block (the A11 prefix of each row)
 __global__ void k_dense_a11(int64_t n1, const int64_t *__restrict__ rowptr, const int64_t *__restrict__ split,
                             const int32_t *__restrict__ colind, const double *__restrict__ val,
@@ -337,7 +357,7 @@ struct Solver {
   double *V = nullptr, *Z = nullptr, *w = nullptr, *b = nullptr, *dx = nullptr, *t1 = nullptr;
   void *V1 = nullptr, *Z1 = nullptr, *w1 = nullptr, *x1 = nullptr, *r1 = nullptr;  // inner, n1-long, FP64 or FP32
   int inner_prec = -1;
-  bool transposed = false;  // preconditioning A^T (adjoint solve): lower block-triangular
+  bool transposed = false;  // adjoint solve: every level holds its transposed Jacobian
   double *d_part = nullptr, *d_dots = nullptr, *d_y = nullptr;
   double *h_dots = nullptr;  // pinned, 3 * ldots + 1
   int ldots = 0;             // stride of the reduction slots: max(m, kin) + 1
@@ -345,6 +365,12 @@ struct Solver {
 };
 
 namespace {
+
+// s = gamma / sigma of the adjoint scaling (1 when either vanishes)
+double adjoint_scale(const mhd_ctx *c) {
+  const double g = c->phys.gamma, sg = c->phys.sigma;
+  return (g > 0.0 && sg > 0.0) ? g / sg : 1.0;
+}
 
 // FGMRES Krylov dimension for a requested restart length
 int krylov_dim(int restart) { return std::max(2, std::min(restart, 2000)); }
@@ -854,44 +880,10 @@ int inner_solve(mhd_ctx *c, int prec) {
   return 0;
 }
 
-// z = P^-T v for the adjoint: P^T = [[A11^T, 0, 0], [A12^T, S_p, 0], [A13^T, 0, S_phi]], so
-// z1 = A11^-T v1 first, then z_p = S_p^-1 (v_p - A12^T z1), z_phi = S_phi^-1 (v_phi - A13^T z1).
-// c->d_val holds A^T here; its p / phi rows only have [u,j] columns.
-int precond_t(mhd_ctx *c, const double *v, double *z, int prec) {
-  Solver *S = c->solver;
-  int rc;
-  const int64_t n = S->n;
-  if (prec == 0) {
-    MHD_HIP(hipMemcpyAsync(S->r1, v, sizeof(double) * S->n1, hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = inner_solve<double>(c, 0))) return rc;
-    MHD_HIP(hipMemcpyAsync(z, S->x1, sizeof(double) * S->n1, hipMemcpyDeviceToDevice, c->stream));
-  } else {
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, v,
-                       (float *)S->r1);
-    if ((rc = inner_solve<float>(c, prec))) return rc;
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1,
-                       (const float *)S->x1, z);
-  }
-  // t1[p, phi rows] = v - A^T z (full rows; only the [u,j] part of z is read)
-  constexpr int LPR = 16;
-  const int64_t nr = n - S->n1;
-  const int grid = (int)((nr * LPR + 255) / 256);
-  hipLaunchKernelGGL((k_csr<double, double, LPR, false>), dim3(grid), dim3(256), 0, c->stream, nr,
-                     c->d_rowptr + S->n1, c->d_colind, c->d_val, z, S->t1 + S->n1);
-  hipLaunchKernelGGL((k_scale_copy<double>), dim3(nblocks(nr)), dim3(TB), 0, c->stream, nr, -1.0, S->t1 + S->n1,
-                     S->t1 + S->n1);
-  hipLaunchKernelGGL((k_axpy<double>), dim3(nblocks(nr)), dim3(TB), 0, c->stream, nr, 1.0, v + S->n1, S->t1 + S->n1);
-  hipLaunchKernelGGL((k_schur<double>), dim3(nblocks((int64_t)c->ncells * 12)), dim3(TB), 0, c->stream, c->ncells,
-                     S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, S->t1, z);
-  MHD_HIP(hipGetLastError());
-  return 0;
-}
-
 // z = P^-1 v (FP64 in/out)
 int precond(mhd_ctx *c, const double *v, double *z, int prec) {
   Solver *S = c->solver;
   int rc;
-  if (S->transposed) return precond_t(c, v, z, prec);
   hipLaunchKernelGGL((k_schur<double>), dim3(nblocks((int64_t)c->ncells * 12)), dim3(TB), 0, c->stream, c->ncells,
                      S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, v, z);
   // rhs1 = v1 - A12 z_p - A13 z_phi (FP64)
@@ -963,6 +955,10 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
         return rc;
       }
       MHD_HIP(hipMemcpyAsync(cc->d_val, cc->d_ebuf, sizeof(double) * cc->nnz, hipMemcpyDeviceToDevice, c->stream));
+      hipLaunchKernelGGL(k_adj_scale, dim3(nblocks(cc->n)), dim3(TB), 0, c->stream, cc->n, cc->d_rowptr,
+                         cc->d_colind, cc->d_val, (int64_t)cc->nfree[0], (int64_t)cc->nfree[0] + cc->nfree[2],
+                         (int64_t)cc->nfree[1], adjoint_scale(c));
+      MHD_HIP(hipGetLastError());
       if (cc->solver) cc->solver->transposed = true;
     }
     if ((rc = precond_setup(cc, prec, cc->d_x))) {
@@ -1315,9 +1311,15 @@ int mhd_dresidual_dparam(mhd_ctx *c, const double *x_free, int32_t param, double
 
 namespace {
 
-// x = A(x_state)^-1 rhs, or A^-T rhs with the transposed preconditioner (lower
-// block-triangular; every level of a V-cycle hierarchy on its transposed
-// Jacobian).  Solver workspaces are rebuilt by the next solve.
+// x = A(x_state)^-1 rhs, or A^-T rhs.  A^T has the saddle-point block
+// structure of A (its (uj, p) block is A's (p, uj) block transposed, which is
+// A's (uj, p) block again; likewise sigma times the (j, phi) block), so the
+// adjoint uses Badia2024's upper block-triangular preconditioner built on A^T
+// itself, every level of a V-cycle hierarchy on its transposed Jacobian.  (The
+// exact transpose of the forward preconditioner, lower block-triangular,
+// stalls with an inexact [u,j] solve on any right-hand side with pressure or
+// potential components: measured from 16 x 16 x 8 up.)  Solver workspaces are
+// rebuilt by the next solve.
 int linear_solve(mhd_ctx *c, const double *x_state, const double *rhs, double *lambda,
                  const mhd_solver_opts *o, mhd_solve_info *info, bool transpose) {
   if (!c || !x_state || !rhs || !lambda || !o || !info) return -1;
@@ -1355,9 +1357,14 @@ int linear_solve(mhd_ctx *c, const double *x_state, const double *rhs, double *l
     d_valT = nullptr;
     return finish(-2);
   }
+  const int64_t nu = c->nfree[0], n1 = (int64_t)c->nfree[0] + c->nfree[2], np = c->nfree[1];
+  const double sc = adjoint_scale(c);
   if (transpose) {
     if ((rc = mhd_launch_transpose_values(c, c->d_val, d_valT, c->stream))) return finish(rc);
     std::swap(c->d_val, d_valT);  // the solver now sees A^T; d_valT holds A
+    hipLaunchKernelGGL(k_adj_scale, dim3(nblocks(n)), dim3(TB), 0, c->stream, n, c->d_rowptr, c->d_colind, c->d_val,
+                       nu, n1, np, sc);
+    MHD_HIP(hipGetLastError());
   }
   MHD_HIP(hipStreamSynchronize(c->stream));
   info->t_jacobian = now() - t;
@@ -1368,6 +1375,8 @@ int linear_solve(mhd_ctx *c, const double *x_state, const double *rhs, double *l
   info->t_setup = now() - t;
   t = now();
   MHD_HIP(hipMemcpyAsync(S->b, rhs, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if (transpose)  // (D^-1 A^T D) mu = D^-1 rhs, lambda = D mu
+    hipLaunchKernelGGL(k_adj_scale_vec, dim3(nblocks(n)), dim3(TB), 0, c->stream, n, nu, n1, np, 1.0 / sc, S->b);
   double bn;
   if ((rc = vnorm(c, S->b, n, &bn))) return finish(rc);
   int kits = 0;
@@ -1375,6 +1384,9 @@ int linear_solve(mhd_ctx *c, const double *x_state, const double *rhs, double *l
   rc = fgmres(c, S->b, S->dx, o->ls_rtol * bn, o->max_krylov, o->precision, &kits, o->verbose, &rfin);
   if (rc < 0) return finish(rc);
   info->residual = rfin;
+  if (transpose)
+    hipLaunchKernelGGL(k_adj_scale_vec, dim3(nblocks(n)), dim3(TB), 0, c->stream, n, nu, n1, np, sc, S->dx);
+  MHD_HIP(hipGetLastError());
   MHD_HIP(hipMemcpyAsync(lambda, S->dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   MHD_HIP(hipStreamSynchronize(c->stream));
   info->t_solve = now() - t;

@@ -164,9 +164,10 @@ void mhd_default_solver_opts(mhd_solver_opts *opts);
  * mhd_dresidual_dparam: out = dR/dtheta at x for theta = gamma (param 0: the
  * Lorentz term -int (j x B).v) or cw (param 1: int_G tau (s.n) n.(grad j).n
  * over every thin-wall face).  dR/dHa = (2 Ha / Re) dR/dgamma in the :cfd scaling.
- * mhd_adjoint_solve: lambda = A(x)^-T rhs with the transposed Badia2024
- * preconditioner (lower block-triangular; an attached V-cycle hierarchy works
- * on the transposed Jacobian of every level).  Then
+ * mhd_adjoint_solve: lambda = A(x)^-T rhs with Badia2024's upper
+ * block-triangular preconditioner built on A^T (which has A's saddle-point
+ * block structure; an attached V-cycle hierarchy works on the transposed
+ * Jacobian of every level).  Then
  * dJ/dtheta = dJ/dtheta|_explicit - lambda . dR/dtheta.
  * mhd_linear_solve: sol = A(x)^-1 rhs with the forward preconditioner; the
  * tangent (forward-sensitivity) form of the same gradient,
