@@ -275,32 +275,40 @@ __global__ void k_schur(int ncells, const int32_t *__restrict__ pfirst, const in
 }
 
 // ---------------------------------------------------------------- patches
-// A_P (column-major, PS x PS) = A11[P, P]; padding rows/cols get the identity.
-__global__ void k_patch_extract(int npatch, int PS, const int32_t *__restrict__ psize, const int32_t *__restrict__ pidx,
-                                const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
-                                const double *__restrict__ val, double *__restrict__ A) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= (int64_t)npatch * PS) return;
-  const int pch = (int)(t / PS), a = (int)(t % PS);
-  double *Ap = A + (int64_t)pch * PS * PS;
+
+// Dense patch matrices from the CSR: one workgroup per patch, the sorted patch
+// ids in LDS; each wave takes whole rows and its lanes binary-search the row's
+// columns among the ids (column-major A_P, identity on the padding; A must be
+// zero on entry).
+__global__ __launch_bounds__(256) void k_patch_extract(int npatch, int PS, const int32_t *__restrict__ psize,
+                                                       const int32_t *__restrict__ pidx,
+                                                       const int64_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ colind,
+                                                       const double *__restrict__ val, double *__restrict__ A) {
+  __shared__ int32_t sid[256];
+  const int pch = blockIdx.x;
   const int sz = psize[pch];
-  if (a >= sz) {
-    Ap[a + (int64_t)a * PS] = 1.0;
-    return;
-  }
   const int32_t *ids = pidx + (int64_t)pch * PS;
-  const int32_t g = ids[a];
-  int64_t p = rowptr[g];
-  const int64_t e = rowptr[g + 1];
-  int b = 0;
-  while (p < e && b < sz) {
-    const int32_t cp = colind[p], cb = ids[b];
-    if (cp < cb) ++p;
-    else if (cp > cb) ++b;
-    else {
-      Ap[a + (int64_t)b * PS] = val[p];
-      ++p;
-      ++b;
+  for (int t = threadIdx.x; t < sz; t += blockDim.x) sid[t] = ids[t];
+  __syncthreads();
+  double *Ap = A + (int64_t)pch * PS * PS;
+  for (int t = sz + threadIdx.x; t < PS; t += blockDim.x) Ap[t + (int64_t)t * PS] = 1.0;
+  if (sz == 0) return;
+  const int32_t first = sid[0], last = sid[sz - 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int a = wave; a < sz; a += nw) {
+    const int32_t g = sid[a];
+    const int64_t e = rowptr[g + 1];
+    for (int64_t p = rowptr[g] + lane; p < e; p += 64) {
+      const int32_t col = colind[p];
+      if (col < first || col > last) continue;
+      int lo = 0, hi = sz;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sid[mid] < col) lo = mid + 1;
+        else hi = mid;
+      }
+      if (sid[lo] == col) Ap[a + (int64_t)lo * PS] = val[p];
     }
   }
 }
@@ -1010,8 +1018,7 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
                        S->d_val32);
   const size_t pelems = (size_t)S->npatch * S->PS * S->PS;
   MHD_HIP(hipMemsetAsync(S->d_pA, 0, sizeof(double) * pelems, c->stream));
-  const int64_t nt = (int64_t)S->npatch * S->PS;
-  hipLaunchKernelGGL(k_patch_extract, dim3(nblocks(nt)), dim3(TB), 0, c->stream, S->npatch, S->PS, S->d_psize,
+  hipLaunchKernelGGL(k_patch_extract, dim3(S->npatch), dim3(256), 0, c->stream, S->npatch, S->PS, S->d_psize,
                      S->d_pidx, c->d_rowptr, c->d_colind, c->d_val, S->d_pA);
   MHD_HIP(hipGetLastError());
   const rocblas_stride st = (rocblas_stride)S->PS * S->PS;
