@@ -113,6 +113,7 @@ __device__ void stage_cell(CellLDS &S, int c, const double *__restrict__ xyz, co
   if (tid < NQ) {
     const int q = tid;
     double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, X[3] = {0, 0, 0};
+#pragma unroll
     for (int v = 0; v < 8; ++v)
       for (int i = 0; i < 3; ++i) {
         const double xv = S.xyz[3 * v + i];
@@ -144,14 +145,21 @@ __device__ void stage_cell(CellLDS &S, int c, const double *__restrict__ xyz, co
   for (int idx = tid; idx < NQ * 4; idx += WG) S.p1[idx / 4][idx % 4] = T->p1[idx / 4][idx % 4];
   for (int idx = tid; idx < NQ * 8; idx += WG) S.q1[idx / 8][idx % 8] = T->q1[idx / 8][idx % 8];
   __syncthreads();
-  // push-forward of the Q2 gradients and the RT basis
-  for (int idx = tid; idx < NQ * 27; idx += WG) {
+  // push-forward of the Q2 gradients and the RT basis (fixed trip counts, so
+  // the reference-table loads of all rounds are issued together)
+#pragma unroll
+  for (int k = 0; k < (NQ * 27 + WG - 1) / WG; ++k) {
+    const int idx = tid + k * WG;
+    if (idx >= NQ * 27) break;
     const int q = idx / 27, a = idx % 27;
     const double *Ji = S.Ji[q];
     const double d0 = T->dq2[q][a][0], d1 = T->dq2[q][a][1], d2 = T->dq2[q][a][2];
     for (int i = 0; i < 3; ++i) S.gphi[q][a][i] = Ji[0 * 3 + i] * d0 + Ji[1 * 3 + i] * d1 + Ji[2 * 3 + i] * d2;
   }
-  for (int idx = tid; idx < NQ * 36; idx += WG) {
+#pragma unroll
+  for (int k = 0; k < (NQ * 36 + WG - 1) / WG; ++k) {
+    const int idx = tid + k * WG;
+    if (idx >= NQ * 36) break;
     const int q = idx / 36, r = idx % 36;
     const double sgn = r < 24 ? S.sg[r >> 2] : 1.0;
     const double *Jp = S.Jp[q];
@@ -160,41 +168,47 @@ __device__ void stage_cell(CellLDS &S, int c, const double *__restrict__ xyz, co
     S.dpsi[q][r] = sgn * T->rtdiv[q][r] * S.idet[q];
   }
   __syncthreads();
-  // state of x at the quadrature points: 27 x 19 scalar reductions
-  for (int idx = tid; idx < NQ * 19; idx += WG) {
-    const int q = idx / 19, k = idx % 19;
+  // state of x at the quadrature points: 27 x 18 scalar reductions of at most
+  // 36 terms (div u is the trace of grad u, taken after the barrier)
+  for (int idx = tid; idx < NQ * 18; idx += WG) {
+    const int k = idx / NQ, q = idx % NQ;  // quantity-major: a wave's lanes share few branches
     double acc = 0.0;
     if (k < 3) {  // u_k
+#pragma unroll
       for (int a = 0; a < 27; ++a) acc += S.xl[k * 27 + a] * S.q2[q][a];
       S.u[q][k] = acc;
     } else if (k < 12) {  // grad u [i][kk]
       const int i = (k - 3) / 3, kk = (k - 3) % 3;
+#pragma unroll
       for (int a = 0; a < 27; ++a) acc += S.xl[kk * 27 + a] * S.gphi[q][a][i];
       S.gu[q][(k - 3)] = acc;
     } else if (k == 12) {
+#pragma unroll
       for (int m = 0; m < 4; ++m) acc += S.xl[OFF_P + m] * S.p1[q][m];
       S.p[q] = acc;
     } else if (k < 16) {
       const int kk = k - 13;
+#pragma unroll
       for (int r = 0; r < 36; ++r) acc += S.xl[OFF_J + r] * S.psi[q][r][kk];
       S.jv[q][kk] = acc;
     } else if (k == 16) {
+#pragma unroll
       for (int r = 0; r < 36; ++r) acc += S.xl[OFF_J + r] * S.dpsi[q][r];
       S.divj[q] = acc;
-    } else if (k == 17) {
+    } else {
+#pragma unroll
       for (int v = 0; v < 8; ++v) acc += S.xl[OFF_PHI + v] * S.q1[q][v];
       S.ph[q] = acc;
-    } else {
-      for (int a = 0; a < 27; ++a)
-        for (int cc = 0; cc < 3; ++cc) acc += S.xl[cc * 27 + a] * S.gphi[q][a][cc];
-      S.divu[q] = acc;
     }
   }
+  __syncthreads();
+  if (tid < NQ) S.divu[tid] = S.gu[tid][0] + S.gu[tid][4] + S.gu[tid][8];
   if (want_al) {
     __syncthreads();
     for (int idx = tid; idx < 4 * NU; idx += WG) {
       const int k = idx / NU, b = idx % NU, cb = b / 27, nb = b % 27;
       double acc = 0.0;
+#pragma unroll
       for (int q = 0; q < NQ; ++q) acc += S.W[q] * S.p1[q][k] * S.gphi[q][nb][cb];
       S.D[k][b] = acc;
     }
@@ -476,25 +490,32 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
       const int a = 16 * ti + li, b = 16 * tj + li;
       const bool av = a < 27, bv = b < 27;
       d4_t Sacc = zero;
-#pragma unroll 3
+#pragma unroll 9
       for (int q = 0; q < NQ; ++q) {
         double A = 0.0, Bv = 0.0;
         if (av) A = lk < 3 ? S.gphi[q][a][lk] : S.q2[q][a];
         if (bv) Bv = S.W[q] * (lk < 3 ? ph.beta * S.gphi[q][b][lk] : (ph.convection ? ph.alpha * S.cphi[q][b] : 0.0));
         Sacc = mfma16(A, Bv, Sacc);
       }
+      // Q2 values of this lane's rows / columns at its 7 quadrature points,
+      // shared by the nine (c, d) blocks
+      double qa[7], qb[7];
+#pragma unroll
+      for (int st = 0; st < 7; ++st) {
+        const int q = 4 * st + lk;
+        qa[st] = (av && q < NQ) ? S.q2[q][a] : 0.0;
+        qb[st] = (bv && q < NQ) ? S.q2[q][b] : 0.0;
+      }
 #pragma unroll 1
       for (int cd = 0; cd < 9; ++cd) {
         const int c = cd / 3, d = cd % 3;
         d4_t acc = c == d ? Sacc : zero;
         if (ph.convection == 2) {
-#pragma unroll 1
-          for (int st = 0; st < 7; ++st) {
-            const int q = 4 * st + lk;
-            const double A = (av && q < NQ) ? S.q2[q][a] : 0.0;
-            const double Bv = (bv && q < NQ) ? S.wN[q][cd] * S.q2[q][b] : 0.0;
-            acc = mfma16(A, Bv, acc);
-          }
+          double wq[7];
+#pragma unroll
+          for (int st = 0; st < 7; ++st) wq[st] = 4 * st + lk < NQ ? S.wN[4 * st + lk][cd] : 0.0;
+#pragma unroll
+          for (int st = 0; st < 7; ++st) acc = mfma16(qa[st], wq[st] * qb[st], acc);
         }
         if (ph.zeta_u != 0.0) {
           const double A = av ? S.D[lk][c * 27 + a] : 0.0;
@@ -510,7 +531,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
       const int r = 16 * ti + li, sidx = 16 * tj + li;
       const bool rv = r < 36, sv = sidx < 36;
       d4_t acc = zero;
-#pragma unroll 3
+#pragma unroll 9
       for (int q = 0; q < NQ; ++q) {
         double A = 0.0, Bv = 0.0;
         if (rv) A = lk < 3 ? S.psi[q][r][lk] : S.dpsi[q][r];
@@ -527,7 +548,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
       const int a = 16 * ti + li, r = 16 * tj + li;
       const bool av = a < 27, rv = r < 36;
       d4_t acc = zero, accT = zero;
-#pragma unroll 1
+#pragma unroll
       for (int st = 0; st < 7; ++st) {
         const int q = 4 * st + lk;
         const double A = (av && q < NQ) ? S.q2[q][a] : 0.0;
@@ -569,6 +590,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
       const int row = t >> 2, k = t & 3, ca = row / 27, na = row % 27;
       if (MODE != SCATTER_EBUF && (S.g[row] < 0 || S.g[OFF_P + k] < 0)) continue;
       double acc = 0.0;
+#pragma unroll
       for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.p1[q][k] * S.gphi[q][na][ca];
       emit<MODE>(S, rk, val, row * ROWS_U_LEN + OFF_P + k, row, OFF_P + k, acc);     // - dp div v
       emit<MODE>(S, rk, val, E_P + k * ROWS_P_LEN + row, OFF_P + k, row, acc);       // - div du q
@@ -576,6 +598,7 @@ __device__ void cell_jacobian_tiles(const CellLDS &S, const MhdPhys &ph, const u
       const int tt = t - NU * 4, r = tt >> 3, v = tt & 7;
       if (MODE != SCATTER_EBUF && (S.g[OFF_J + r] < 0 || S.g[OFF_PHI + v] < 0)) continue;
       double acc = 0.0;
+#pragma unroll
       for (int q = 0; q < NQ; ++q) acc -= S.W[q] * S.q1[q][v] * S.dpsi[q][r];
       emit<MODE>(S, rk, val, E_J + r * ROWS_J_LEN + NU + NJ + v, OFF_J + r, OFF_PHI + v, ph.sigma * acc);  // - sigma dphi div s
       emit<MODE>(S, rk, val, E_PHI + v * ROWS_PHI_LEN + r, OFF_PHI + v, OFF_J + r, acc);                   // - div dj phi_t

@@ -129,6 +129,7 @@ struct CoarseLink {
   int coarse_iters = 10;           // Schwarz-GMRES steps on the coarsest level; 0: dense inverse
   // coarsest level solved exactly: explicit inverse of its [u,j] block (column-major)
   double *d_Ainv = nullptr;        // [n1c][n1c]
+  double *d_LU = nullptr;          // [n1c][n1c] LU factors (set-up scratch)
   int32_t *d_piv = nullptr, *d_info = nullptr;
   double *d_rhs = nullptr;         // [2 n1c] FP64 staging of FP32 vectors
   double omega = 0.2;              // damping of the Schwarz smoother inside the V-cycle

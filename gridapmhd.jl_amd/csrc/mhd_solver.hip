@@ -221,6 +221,12 @@ __global__ void k_adj_scale_vec(int64_t n, int64_t nu, int64_t n1, int64_t np, d
     if ((i >= nu && i < n1) || i >= n1 + np) x[i] *= f;
 }
 
+// D = diag(v) on an n x n column-major matrix (off-diagonal untouched)
+__global__ void k_set_diag(int64_t n, double v, double *__restrict__ D) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    D[i + i * n] = v;
+}
+
 // dense column-major copy of the [u,j] block (the A11 prefix of each row)
 __global__ void k_dense_a11(int64_t n1, const int64_t *__restrict__ rowptr, const int64_t *__restrict__ split,
                             const int32_t *__restrict__ colind, const double *__restrict__ val,
@@ -974,21 +980,25 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
       return rc;
     }
     if (L->coarse_iters == 0 && !cc->coarse) {
-      // exact coarsest solve: dense LU and explicit inverse of its [u,j] block
+      // exact coarsest solve: dense LU of its [u,j] block, then the explicit
+      // inverse as the LU solve against the identity (trsm-based, GEMM-rich)
       const int64_t m = L->n1c;
       if (!L->d_Ainv) {
         MHD_HIP(hipMalloc(&L->d_Ainv, sizeof(double) * (size_t)(m * m)));
+        MHD_HIP(hipMalloc(&L->d_LU, sizeof(double) * (size_t)(m * m)));
         MHD_HIP(hipMalloc(&L->d_piv, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)));
         MHD_HIP(hipMalloc(&L->d_info, sizeof(int32_t)));
       }
-      MHD_HIP(hipMemsetAsync(L->d_Ainv, 0, sizeof(double) * (size_t)(m * m), c->stream));
+      MHD_HIP(hipMemsetAsync(L->d_LU, 0, sizeof(double) * (size_t)(m * m), c->stream));
       hipLaunchKernelGGL(k_dense_a11, dim3(nblocks(m)), dim3(TB), 0, c->stream, m, cc->d_rowptr,
-                         cc->solver->d_split, cc->d_colind, cc->d_val, L->d_Ainv);
+                         cc->solver->d_split, cc->d_colind, cc->d_val, L->d_LU);
+      MHD_HIP(hipMemsetAsync(L->d_Ainv, 0, sizeof(double) * (size_t)(m * m), c->stream));
+      hipLaunchKernelGGL(k_set_diag, dim3(nblocks(m)), dim3(TB), 0, c->stream, m, 1.0, L->d_Ainv);
       MHD_HIP(hipGetLastError());
-      if (rocsolver_dgetrf(S->rb, (rocblas_int)m, (rocblas_int)m, L->d_Ainv, (rocblas_int)m, L->d_piv, L->d_info) !=
+      if (rocsolver_dgetrf(S->rb, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m, L->d_piv, L->d_info) !=
               rocblas_status_success ||
-          rocsolver_dgetri(S->rb, (rocblas_int)m, L->d_Ainv, (rocblas_int)m, L->d_piv, L->d_info) !=
-              rocblas_status_success) {
+          rocsolver_dgetrs(S->rb, rocblas_operation_none, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m,
+                           L->d_piv, L->d_Ainv, (rocblas_int)m) != rocblas_status_success) {
         c->err = "rocsolver LU of the coarsest level failed";
         return -2;
       }
@@ -999,7 +1009,6 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
         c->err = "coarsest [u,j] block is singular (getrf info " + std::to_string(info) + ")";
         return -3;
       }
-
     }
   }
   const double ap = -1.0 / (c->phys.beta + c->phys.zeta_u);
@@ -1134,7 +1143,7 @@ void mhd_coarse_free(mhd_ctx *c) {
   if (!c || !c->coarse) return;
   CoarseLink *L = c->coarse;
   void *ptrs[] = {L->d_Prow, L->d_Trow, L->d_Pcol, L->d_Tcol, L->d_Pval, L->d_Tval, L->d_Pval32, L->d_Tval32,
-                  L->d_inj, L->rr, L->rc, L->ec, L->d_Ainv, L->d_piv, L->d_info, L->d_rhs, L->d_ab};
+                  L->d_inj, L->rr, L->rc, L->ec, L->d_Ainv, L->d_LU, L->d_piv, L->d_info, L->d_rhs, L->d_ab};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (L->ctx) mhd_destroy(L->ctx);
