@@ -127,11 +127,10 @@ struct CoarseLink {
   float *d_Pval32 = nullptr, *d_Tval32 = nullptr;
   int32_t *d_inj = nullptr;        // [nuc] fine u dof of every coarse u dof
   int coarse_iters = 10;           // Schwarz-GMRES steps on the coarsest level; 0: dense inverse
-  // coarsest level solved exactly: explicit inverse of its [u,j] block (column-major)
+  // coarsest level solved exactly: explicit inverse of its [u,j] block (row-major)
   double *d_Ainv = nullptr;        // [n1c][n1c]
   double *d_LU = nullptr;          // [n1c][n1c] LU factors (set-up scratch)
   int32_t *d_piv = nullptr, *d_info = nullptr;
-  double *d_rhs = nullptr;         // [2 n1c] FP64 staging of FP32 vectors
   double omega = 0.2;              // damping of the Schwarz smoother inside the V-cycle
   int step_mode = 1;               // coarse correction step: 0 full, 1 minimal residual
   void *rr = nullptr;              // [3 (n1 + 1)] fine residual, A11 yc, yc (T)

@@ -319,6 +319,29 @@ __global__ __launch_bounds__(256) void k_patch_extract(int npatch, int PS, const
   }
 }
 
+// y = A^-1 x with the explicit inverse stored row-major: one wave per row,
+// coalesced 8-byte loads of the row, FP64 accumulation, T in and out.
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_rowdot(int64_t m, const double *__restrict__ Ainv,
+                                                      const T *__restrict__ x, T *__restrict__ y) {
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= m) return;  // uniform over the wave
+  const double *a = Ainv + row * m;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t j = lane;
+  for (; j + 192 < m; j += 256) {
+    s0 += a[j] * (double)x[j];
+    s1 += a[j + 64] * (double)x[j + 64];
+    s2 += a[j + 128] * (double)x[j + 128];
+    s3 += a[j + 192] * (double)x[j + 192];
+  }
+  for (; j < m; j += 64) s0 += a[j] * (double)x[j];
+  double v = (s0 + s1) + (s2 + s3);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (lane == 0) y[row] = (T)v;
+}
+
 // y[P] += w A_P^-1 r[P]  (one workgroup per patch, inverse column-major)
 template <class MP, class T>
 __global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const int32_t *__restrict__ psize,
@@ -740,30 +763,15 @@ int spmv_a11(mhd_ctx *c, const T *x, T *y, int prec) {
 template <class T>
 int inner_solve(mhd_ctx *c, int prec);
 
-// ec = A11c^-1 rc on the coarsest level with the explicit FP64 inverse (GEMV);
-// FP32 vectors are widened for it, since the coarsest block is the worst
-// conditioned one and its correction reaches every level above.
+// ec = A11c^-1 rc on the coarsest level with the explicit FP64 inverse;
+// FP32 vectors are widened inside the product, since the coarsest block is
+// the worst conditioned one and its correction reaches every level above.
 template <class T>
-int coarse_direct(mhd_ctx *c, const T *rc, T *ec, int prec) {
-  Solver *S = c->solver;
+int coarse_direct(mhd_ctx *c, const T *rc, T *ec) {
   CoarseLink *L = c->coarse;
-  const rocblas_int n = (rocblas_int)L->n1c;
-  const double one = 1.0, zero = 0.0;
-  rocblas_status st;
-  if (std::is_same<T, double>::value) {
-    st = rocblas_dgemv(S->rb, rocblas_operation_none, n, n, &one, L->d_Ainv, n, (const double *)rc, 1, &zero,
-                       (double *)ec, 1);
-  } else {
-    if (!L->d_rhs) MHD_HIP(hipMalloc(&L->d_rhs, sizeof(double) * 2 * (size_t)std::max<int64_t>(L->n1c, 1)));
-    double *x = L->d_rhs, *y = L->d_rhs + L->n1c;
-    hipLaunchKernelGGL((k_convert<T, double>), dim3(nblocks(L->n1c)), dim3(TB), 0, c->stream, L->n1c, rc, x);
-    st = rocblas_dgemv(S->rb, rocblas_operation_none, n, n, &one, L->d_Ainv, n, x, 1, &zero, y, 1);
-    hipLaunchKernelGGL((k_convert<double, T>), dim3(nblocks(L->n1c)), dim3(TB), 0, c->stream, L->n1c, y, ec);
-  }
-  if (st != rocblas_status_success) {
-    c->err = "coarsest-level GEMV failed";
-    return -2;
-  }
+  const int64_t n = L->n1c;
+  hipLaunchKernelGGL((k_dense_rowdot<T>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c->stream, n, L->d_Ainv, rc,
+                     ec);
   MHD_HIP(hipGetLastError());
   return 0;
 }
@@ -811,7 +819,7 @@ int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
   if (cc->coarse) {
     if ((rc = vcycle<T>(cc, rcv, ec, prec))) return rc;
   } else if (L->coarse_iters == 0) {
-    if ((rc = coarse_direct<T>(c, rcv, ec, prec))) return rc;
+    if ((rc = coarse_direct<T>(c, rcv, ec))) return rc;
   } else {
     MHD_HIP(hipMemcpyAsync(SC->r1, rcv, sizeof(T) * L->n1c, hipMemcpyDeviceToDevice, c->stream));
     if ((rc = inner_solve<T>(cc, prec))) return rc;
@@ -981,7 +989,8 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
     }
     if (L->coarse_iters == 0 && !cc->coarse) {
       // exact coarsest solve: dense LU of its [u,j] block, then the explicit
-      // inverse as the LU solve against the identity (trsm-based, GEMM-rich)
+      // inverse as the LU solve A^T X = I (trsm-based, GEMM-rich): X = A^-T
+      // column-major is A^-1 row-major, the layout of the row-dot product
       const int64_t m = L->n1c;
       if (!L->d_Ainv) {
         MHD_HIP(hipMalloc(&L->d_Ainv, sizeof(double) * (size_t)(m * m)));
@@ -997,7 +1006,7 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
       MHD_HIP(hipGetLastError());
       if (rocsolver_dgetrf(S->rb, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m, L->d_piv, L->d_info) !=
               rocblas_status_success ||
-          rocsolver_dgetrs(S->rb, rocblas_operation_none, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m,
+          rocsolver_dgetrs(S->rb, rocblas_operation_transpose, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m,
                            L->d_piv, L->d_Ainv, (rocblas_int)m) != rocblas_status_success) {
         c->err = "rocsolver LU of the coarsest level failed";
         return -2;
@@ -1143,7 +1152,7 @@ void mhd_coarse_free(mhd_ctx *c) {
   if (!c || !c->coarse) return;
   CoarseLink *L = c->coarse;
   void *ptrs[] = {L->d_Prow, L->d_Trow, L->d_Pcol, L->d_Tcol, L->d_Pval, L->d_Tval, L->d_Pval32, L->d_Tval32,
-                  L->d_inj, L->rr, L->rc, L->ec, L->d_Ainv, L->d_LU, L->d_piv, L->d_info, L->d_rhs, L->d_ab};
+                  L->d_inj, L->rr, L->rc, L->ec, L->d_Ainv, L->d_LU, L->d_piv, L->d_info, L->d_ab};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (L->ctx) mhd_destroy(L->ctx);
