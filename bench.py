@@ -42,10 +42,10 @@ FP64_PEAK_TFS = 78.6
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) over
 # `bench.py --kernels-only`, each scaled by what the counter reads for a 1 GiB
 # device-to-device copy (FETCH_SIZE x 2, WRITE_SIZE x 1).
-C2_TRAFFIC = {"smoother_f32": 6.739e9, "smoother_f64": 1.3924e10, "spmv_f64": 5.852e9, "jacobian": 1.3635e10}
+C2_TRAFFIC = {"smoother_f32": 6.739e9, "smoother_f64": 1.3914e10, "spmv_f64": 5.991e9, "jacobian": 1.3641e10}
 # k_cell_jacobian at 64 x 64 x 8: SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 flops and
 # SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE per XCD x 1024 SIMDs), same runs.
-C2_JACOBIAN_MFMA = {"executed_flops": 5.979e10, "mfma_busy_frac": 0.176, "cell_kernel_ms": 4.36}
+C2_JACOBIAN_MFMA = {"executed_flops": 5.979e10, "mfma_busy_frac": 0.231, "cell_kernel_ms": 3.341}
 
 
 def log(msg):
