@@ -238,6 +238,10 @@ void mhd_destroy(mhd_ctx *c) {
                   c->d_color_cells, c->d_crow, c->d_centry, c->d_ebuf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->setup_stream) (void)hipStreamSynchronize(c->setup_stream);
+  if (c->setup_fork) (void)hipEventDestroy(c->setup_fork);
+  if (c->setup_join) (void)hipEventDestroy(c->setup_join);
+  if (c->setup_stream) (void)hipStreamDestroy(c->setup_stream);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -114,6 +114,9 @@ struct mhd_ctx {
   struct Solver *solver = nullptr;
   // coarser level of the [u,j] block solve (owned), see mhd_attach_coarse
   struct CoarseLink *coarse = nullptr;
+  // the coarse levels' set-up runs on this stream beside the fine level's
+  hipStream_t setup_stream = nullptr;
+  hipEvent_t setup_fork = nullptr, setup_join = nullptr;
 };
 
 // Link from a level to the next coarser one: prolongation P (n1 x n1c CSR),

@@ -949,77 +949,16 @@ int coarse_alloc(mhd_ctx *c, int prec) {
   return 0;
 }
 
-// Per-Jacobian set-up: Schur scalings, FP32 copy, patch factorisation; on
-// every coarser level the u state is injected from `d_state` and the [u,j]
-// block is re-assembled with the same kernels (rediscretisation).
-int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
+// Per-Jacobian set-up of one level: Schur scalings, FP32 copy and patch
+// factorisation on c->stream, then the coarser levels on their own stream:
+// the u state is injected from `d_state` and the [u,j] block re-assembled
+// with the same kernels (rediscretisation).  This level's work is launched
+// first, so that with the coarse chain on a second stream the two overlap.
+int level_setup(mhd_ctx *c, int prec, const double *d_state) {
   Solver *S = c->solver;
   int rc;
   if ((rc = inner_alloc(c, prec))) return rc;
-  if (c->coarse) {
-    CoarseLink *L = c->coarse;
-    mhd_ctx *cc = L->ctx;
-    if ((rc = coarse_alloc(c, prec))) return rc;
-    cc->phys = c->phys;
-    hipLaunchKernelGGL(k_inject, dim3(nblocks(cc->n)), dim3(TB), 0, c->stream, cc->n, L->nuc, L->d_inj, d_state,
-                       cc->d_x);
-    MHD_HIP(hipGetLastError());
-    if ((rc = mhd_launch_jacobian(cc, cc->d_x, cc->d_val, c->stream))) {
-      c->err = "coarse level: " + cc->err;
-      return rc;
-    }
-    if (S->transposed) {
-      // adjoint: the coarse levels carry A_c^T (same transfers P, P^T); the
-      // element buffer, free after the assembly, holds the transposed copy
-      if (!cc->d_ebuf) MHD_HIP(hipMalloc(&cc->d_ebuf, sizeof(double) * (size_t)std::max(cc->ncells, 1) * NENT));
-      if ((rc = mhd_launch_transpose_values(cc, cc->d_val, cc->d_ebuf, c->stream))) {
-        c->err = "coarse level: " + cc->err;
-        return rc;
-      }
-      MHD_HIP(hipMemcpyAsync(cc->d_val, cc->d_ebuf, sizeof(double) * cc->nnz, hipMemcpyDeviceToDevice, c->stream));
-      hipLaunchKernelGGL(k_adj_scale, dim3(nblocks(cc->n)), dim3(TB), 0, c->stream, cc->n, cc->d_rowptr,
-                         cc->d_colind, cc->d_val, (int64_t)cc->nfree[0], (int64_t)cc->nfree[0] + cc->nfree[2],
-                         (int64_t)cc->nfree[1], adjoint_scale(c));
-      MHD_HIP(hipGetLastError());
-      if (cc->solver) cc->solver->transposed = true;
-    }
-    if ((rc = precond_setup(cc, prec, cc->d_x))) {
-      c->err = "coarse level: " + cc->err;
-      return rc;
-    }
-    if (L->coarse_iters == 0 && !cc->coarse) {
-      // exact coarsest solve: dense LU of its [u,j] block, then the explicit
-      // inverse as the LU solve A^T X = I (trsm-based, GEMM-rich): X = A^-T
-      // column-major is A^-1 row-major, the layout of the row-dot product
-      const int64_t m = L->n1c;
-      if (!L->d_Ainv) {
-        MHD_HIP(hipMalloc(&L->d_Ainv, sizeof(double) * (size_t)(m * m)));
-        MHD_HIP(hipMalloc(&L->d_LU, sizeof(double) * (size_t)(m * m)));
-        MHD_HIP(hipMalloc(&L->d_piv, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)));
-        MHD_HIP(hipMalloc(&L->d_info, sizeof(int32_t)));
-      }
-      MHD_HIP(hipMemsetAsync(L->d_LU, 0, sizeof(double) * (size_t)(m * m), c->stream));
-      hipLaunchKernelGGL(k_dense_a11, dim3(nblocks(m)), dim3(TB), 0, c->stream, m, cc->d_rowptr,
-                         cc->solver->d_split, cc->d_colind, cc->d_val, L->d_LU);
-      MHD_HIP(hipMemsetAsync(L->d_Ainv, 0, sizeof(double) * (size_t)(m * m), c->stream));
-      hipLaunchKernelGGL(k_set_diag, dim3(nblocks(m)), dim3(TB), 0, c->stream, m, 1.0, L->d_Ainv);
-      MHD_HIP(hipGetLastError());
-      if (rocsolver_dgetrf(S->rb, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m, L->d_piv, L->d_info) !=
-              rocblas_status_success ||
-          rocsolver_dgetrs(S->rb, rocblas_operation_transpose, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m,
-                           L->d_piv, L->d_Ainv, (rocblas_int)m) != rocblas_status_success) {
-        c->err = "rocsolver LU of the coarsest level failed";
-        return -2;
-      }
-      int32_t info = 0;
-      MHD_HIP(hipMemcpyAsync(&info, L->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-      MHD_HIP(hipStreamSynchronize(c->stream));
-      if (info != 0) {
-        c->err = "coarsest [u,j] block is singular (getrf info " + std::to_string(info) + ")";
-        return -3;
-      }
-    }
-  }
+  if (c->coarse && (rc = coarse_alloc(c, prec))) return rc;
   const double ap = -1.0 / (c->phys.beta + c->phys.zeta_u);
   const double aphi = -1.0 / (1.0 + c->phys.zeta_j);
   if (ap != S->alpha_p || aphi != S->alpha_phi) {
@@ -1054,6 +993,102 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
     hipLaunchKernelGGL(k_to_bf16, dim3(nblocks((int64_t)pelems)), dim3(TB), 0, c->stream, (int64_t)pelems, S->d_pA,
                        S->d_pA16);
   MHD_HIP(hipGetLastError());
+  if (!c->coarse) return 0;
+  CoarseLink *L = c->coarse;
+  mhd_ctx *cc = L->ctx;
+  hipStream_t cs = cc->stream;
+  cc->phys = c->phys;
+  hipLaunchKernelGGL(k_inject, dim3(nblocks(cc->n)), dim3(TB), 0, cs, cc->n, L->nuc, L->d_inj, d_state, cc->d_x);
+  MHD_HIP(hipGetLastError());
+  if ((rc = mhd_launch_jacobian(cc, cc->d_x, cc->d_val, cs))) {
+    c->err = "coarse level: " + cc->err;
+    return rc;
+  }
+  if (S->transposed) {
+    // adjoint: the coarse levels carry A_c^T (same transfers P, P^T); the
+    // element buffer, free after the assembly, holds the transposed copy
+    if (!cc->d_ebuf) MHD_HIP(hipMalloc(&cc->d_ebuf, sizeof(double) * (size_t)std::max(cc->ncells, 1) * NENT));
+    if ((rc = mhd_launch_transpose_values(cc, cc->d_val, cc->d_ebuf, cs))) {
+      c->err = "coarse level: " + cc->err;
+      return rc;
+    }
+    MHD_HIP(hipMemcpyAsync(cc->d_val, cc->d_ebuf, sizeof(double) * cc->nnz, hipMemcpyDeviceToDevice, cs));
+    hipLaunchKernelGGL(k_adj_scale, dim3(nblocks(cc->n)), dim3(TB), 0, cs, cc->n, cc->d_rowptr, cc->d_colind,
+                       cc->d_val, (int64_t)cc->nfree[0], (int64_t)cc->nfree[0] + cc->nfree[2], (int64_t)cc->nfree[1],
+                       adjoint_scale(c));
+    MHD_HIP(hipGetLastError());
+    if (cc->solver) cc->solver->transposed = true;
+  }
+  if ((rc = level_setup(cc, prec, cc->d_x))) {
+    c->err = "coarse level: " + cc->err;
+    return rc;
+  }
+  if (L->coarse_iters == 0 && !cc->coarse) {
+    // exact coarsest solve: dense LU of its [u,j] block, then the explicit
+    // inverse as the LU solve A^T X = I (trsm-based, GEMM-rich): X = A^-T
+    // column-major is A^-1 row-major, the layout of the row-dot product
+    const int64_t m = L->n1c;
+    rocblas_handle rb = cc->solver->rb;  // bound to cs
+    if (!L->d_Ainv) {
+      MHD_HIP(hipMalloc(&L->d_Ainv, sizeof(double) * (size_t)(m * m)));
+      MHD_HIP(hipMalloc(&L->d_LU, sizeof(double) * (size_t)(m * m)));
+      MHD_HIP(hipMalloc(&L->d_piv, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)));
+      MHD_HIP(hipMalloc(&L->d_info, sizeof(int32_t)));
+    }
+    MHD_HIP(hipMemsetAsync(L->d_LU, 0, sizeof(double) * (size_t)(m * m), cs));
+    hipLaunchKernelGGL(k_dense_a11, dim3(nblocks(m)), dim3(TB), 0, cs, m, cc->d_rowptr, cc->solver->d_split,
+                       cc->d_colind, cc->d_val, L->d_LU);
+    MHD_HIP(hipMemsetAsync(L->d_Ainv, 0, sizeof(double) * (size_t)(m * m), cs));
+    hipLaunchKernelGGL(k_set_diag, dim3(nblocks(m)), dim3(TB), 0, cs, m, 1.0, L->d_Ainv);
+    MHD_HIP(hipGetLastError());
+    if (rocsolver_dgetrf(rb, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m, L->d_piv, L->d_info) !=
+            rocblas_status_success ||
+        rocsolver_dgetrs(rb, rocblas_operation_transpose, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m,
+                         L->d_piv, L->d_Ainv, (rocblas_int)m) != rocblas_status_success) {
+      c->err = "rocsolver LU of the coarsest level failed";
+      return -2;
+    }
+    int32_t info = 0;
+    MHD_HIP(hipMemcpyAsync(&info, L->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, cs));
+    MHD_HIP(hipStreamSynchronize(cs));
+    if (info != 0) {
+      c->err = "coarsest [u,j] block is singular (getrf info " + std::to_string(info) + ")";
+      return -3;
+    }
+  }
+  return 0;
+}
+
+// Points every coarser level (its launches and its rocBLAS handle) at stream s.
+void chain_stream(mhd_ctx *c, hipStream_t s) {
+  for (mhd_ctx *q = c->coarse ? c->coarse->ctx : nullptr; q; q = q->coarse ? q->coarse->ctx : nullptr) {
+    q->stream = s;
+    if (q->solver && q->solver->rb) rocblas_set_stream(q->solver->rb, s);
+  }
+}
+
+// Set-up of the whole hierarchy: the coarse chain runs on a second stream,
+// forked after the work already queued on c->stream and joined back into it,
+// so the fine level's batched patch factorisation overlaps the coarse
+// levels' assembly, patch factorisations and dense coarsest LU.
+int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
+  if (!c->coarse) return level_setup(c, prec, d_state);
+  if (!c->setup_stream) {
+    MHD_HIP(hipStreamCreateWithFlags(&c->setup_stream, hipStreamNonBlocking));
+    MHD_HIP(hipEventCreateWithFlags(&c->setup_fork, hipEventDisableTiming));
+    MHD_HIP(hipEventCreateWithFlags(&c->setup_join, hipEventDisableTiming));
+  }
+  MHD_HIP(hipEventRecord(c->setup_fork, c->stream));
+  MHD_HIP(hipStreamWaitEvent(c->setup_stream, c->setup_fork, 0));
+  chain_stream(c, c->setup_stream);
+  const int rc = level_setup(c, prec, d_state);
+  // join also after an error, so that the chain is back on c->stream
+  const hipError_t e1 = hipEventRecord(c->setup_join, c->setup_stream);
+  const hipError_t e2 = hipStreamWaitEvent(c->stream, c->setup_join, 0);
+  chain_stream(c, c->stream);
+  if (rc) return rc;
+  MHD_HIP(e1);
+  MHD_HIP(e2);
   return 0;
 }
 
