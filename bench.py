@@ -242,7 +242,7 @@ def main():
     ap.add_argument("--asm-reps", type=int, default=10)
     ap.add_argument("--mixed", type=int, default=1, choices=[1, 2], help="mixed mode: 1 FP32 / 2 BF16 patch blocks")
     ap.add_argument("--restart", type=int, default=200)
-    ap.add_argument("--inner", type=int, default=3, help="flexible-GMRES steps of the inexact [u,j] solve")
+    ap.add_argument("--inner", type=int, default=4, help="flexible-GMRES steps of the inexact [u,j] solve")
     ap.add_argument("--levels", default="auto", help="levels of the [u,j] V-cycle (1 = Schwarz only; auto: 5 at C2)")
     ap.add_argument("--coarse-iters", type=int, default=0, help="0: exact coarsest solve")
     ap.add_argument("--weight", type=float, default=0.2, help="Schwarz smoother damping in the V-cycle")
