@@ -342,11 +342,11 @@ __global__ __launch_bounds__(256) void k_dense_rowdot(int64_t m, const double *_
   if (lane == 0) y[row] = (T)v;
 }
 
-// y[P] += w A_P^-1 r[P]  (one workgroup per patch, inverse column-major)
+// part[P] = A_P^-1 r[P]  (one workgroup per patch, inverse column-major)
 template <class MP, class T>
 __global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const int32_t *__restrict__ psize,
                                                      const int32_t *__restrict__ pidx, const MP *__restrict__ Ainv,
-                                                     const T *__restrict__ r, T *__restrict__ y, double w) {
+                                                     const T *__restrict__ r, double *__restrict__ part) {
   __shared__ double rl[256];
   const int pch = blockIdx.x;
   const int sz = psize[pch];
@@ -365,8 +365,21 @@ __global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const i
       a3 += todbl(Ap[(int64_t)(b + 3) * PS]) * rl[b + 3];
     }
     for (; b < sz; ++b) a0 += todbl(Ap[(int64_t)b * PS]) * rl[b];
-    atomicAdd(&y[ids[t]], (T)(w * ((a0 + a1) + (a2 + a3))));
+    part[(int64_t)pch * PS + t] = (a0 + a1) + (a2 + a3);
   }
+}
+
+// y[i] (+)= w sum over the patches P holding dof i of part[P][i], summed in
+// FP64 in patch order: no atomics, the same result in every run
+template <class T>
+__global__ void k_patch_gather(int64_t n, const int32_t *__restrict__ dptr, const int32_t *__restrict__ dent,
+                               const double *__restrict__ part, double w, T *__restrict__ y, int accumulate) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int32_t e = dptr[i]; e < dptr[i + 1]; ++e) s += part[dent[e]];
+  s *= w;
+  y[i] = accumulate ? (T)((double)y[i] + s) : (T)s;
 }
 
 }  // namespace
@@ -382,6 +395,9 @@ struct Solver {
   // patches
   int npatch = 0, PS = 0;
   int32_t *d_psize = nullptr, *d_pidx = nullptr;
+  // dof -> patch rows holding it (patch * PS + local index), in patch order
+  int32_t *d_dptr = nullptr, *d_dent = nullptr;
+  double *d_ppart = nullptr;  // [npatch * PS] per-patch products
   double *d_pA = nullptr;
   float *d_pA32 = nullptr;
   __hip_bfloat16 *d_pA16 = nullptr;
@@ -575,6 +591,23 @@ int solver_init(mhd_ctx *c, const mhd_solver_opts *o) {
   if ((rc = dmalloc(c, &S->d_pidx, pidx.size()))) return rc;
   MHD_HIP(hipMemcpy(S->d_psize, psize.data(), sizeof(int32_t) * psize.size(), hipMemcpyHostToDevice));
   MHD_HIP(hipMemcpy(S->d_pidx, pidx.data(), sizeof(int32_t) * pidx.size(), hipMemcpyHostToDevice));
+  {
+    // the smoother sums the patch products of a dof in this fixed order, so
+    // its result does not depend on the order the patches finish in
+    std::vector<int32_t> dptr(S->n1 + 1, 0), dent;
+    for (int p = 0; p < S->npatch; ++p)
+      for (int a = 0; a < psize[p]; ++a) ++dptr[pidx[(size_t)p * S->PS + a] + 1];
+    for (int64_t i = 0; i < S->n1; ++i) dptr[i + 1] += dptr[i];
+    dent.resize(dptr[S->n1]);
+    std::vector<int32_t> fill(dptr.begin(), dptr.end() - 1);
+    for (int p = 0; p < S->npatch; ++p)
+      for (int a = 0; a < psize[p]; ++a) dent[fill[pidx[(size_t)p * S->PS + a]]++] = p * S->PS + a;
+    if ((rc = dmalloc(c, &S->d_dptr, dptr.size()))) return rc;
+    if ((rc = dmalloc(c, &S->d_dent, dent.size()))) return rc;
+    if ((rc = dmalloc(c, &S->d_ppart, (size_t)S->npatch * S->PS))) return rc;
+    MHD_HIP(hipMemcpy(S->d_dptr, dptr.data(), sizeof(int32_t) * dptr.size(), hipMemcpyHostToDevice));
+    MHD_HIP(hipMemcpy(S->d_dent, dent.data(), sizeof(int32_t) * dent.size(), hipMemcpyHostToDevice));
+  }
   const size_t pelems = (size_t)S->npatch * S->PS * S->PS;
   if ((rc = dmalloc(c, &S->d_pA, pelems))) return rc;
   if ((rc = dmalloc(c, &S->d_ipiv, (size_t)S->npatch * S->PS))) return rc;
@@ -729,20 +762,26 @@ void hess_solve(int k, const std::vector<double> &H, int ldh, double beta, std::
 // Additive Schwarz over the vertex patches: y = sum_P R_P^T A_P^-1 R_P r
 // y (+)= w sum_P R_P^T A_P^-1 R_P r
 template <class T>
-int patch_as(mhd_ctx *c, const T *r, T *y, int prec, double w = 1.0, bool accumulate = false) {
+int patch_as_on(mhd_ctx *c, const T *r, T *y, int prec, double w, bool accumulate, hipStream_t st) {
   Solver *S = c->solver;
-  if (!accumulate) hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, (T)0, y);
   if (prec == 0)
-    hipLaunchKernelGGL((k_patch_apply<double, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch, S->PS,
-                       S->d_psize, S->d_pidx, S->d_pA, r, y, w);
+    hipLaunchKernelGGL((k_patch_apply<double, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS, S->d_psize,
+                       S->d_pidx, S->d_pA, r, S->d_ppart);
   else if (prec == 1)
-    hipLaunchKernelGGL((k_patch_apply<float, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch, S->PS,
-                       S->d_psize, S->d_pidx, S->d_pA32, r, y, w);
+    hipLaunchKernelGGL((k_patch_apply<float, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS, S->d_psize,
+                       S->d_pidx, S->d_pA32, r, S->d_ppart);
   else
-    hipLaunchKernelGGL((k_patch_apply<__hip_bfloat16, T>), dim3(S->npatch), dim3(256), 0, c->stream, S->npatch,
-                       S->PS, S->d_psize, S->d_pidx, S->d_pA16, r, y, w);
+    hipLaunchKernelGGL((k_patch_apply<__hip_bfloat16, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS,
+                       S->d_psize, S->d_pidx, S->d_pA16, r, S->d_ppart);
+  hipLaunchKernelGGL((k_patch_gather<T>), dim3(nblocks(S->n1)), dim3(TB), 0, st, S->n1, S->d_dptr, S->d_dent,
+                     S->d_ppart, w, y, accumulate ? 1 : 0);
   MHD_HIP(hipGetLastError());
   return 0;
+}
+
+template <class T>
+int patch_as(mhd_ctx *c, const T *r, T *y, int prec, double w = 1.0, bool accumulate = false) {
+  return patch_as_on<T>(c, r, y, prec, w, accumulate, c->stream);
 }
 
 template <class T>
@@ -1174,7 +1213,8 @@ void mhd_solver_free(mhd_ctx *c) {
   Solver *S = c->solver;
   void *ptrs[] = {S->d_split, S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, S->d_psize, S->d_pidx, S->d_pA,
                   S->d_pA32, S->d_pA16, S->d_ipiv, S->d_info, S->d_val32, S->V, S->Z, S->w, S->b, S->dx, S->t1,
-                  S->V1, S->Z1, S->w1, S->x1, S->r1, S->d_part, S->d_dots, S->d_y};
+                  S->V1, S->Z1, S->w1, S->x1, S->r1, S->d_part, S->d_dots, S->d_y, S->d_dptr, S->d_dent,
+                  S->d_ppart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (S->h_dots) (void)hipHostFree(S->h_dots);
@@ -1327,17 +1367,8 @@ int mhd_smoother_apply_d(mhd_ctx *c, const void *d_r, void *d_y, void *stream) {
   MHD_HIP(hipSetDevice(c->device));
   Solver *S = c->solver;
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  if (S->inner_prec == 0)
-    hipLaunchKernelGGL((k_patch_apply<double, double>), dim3(S->npatch), dim3(256), 0, s, S->npatch, S->PS,
-                       S->d_psize, S->d_pidx, S->d_pA, (const double *)d_r, (double *)d_y, 1.0);
-  else if (S->inner_prec == 1)
-    hipLaunchKernelGGL((k_patch_apply<float, float>), dim3(S->npatch), dim3(256), 0, s, S->npatch, S->PS,
-                       S->d_psize, S->d_pidx, S->d_pA32, (const float *)d_r, (float *)d_y, 1.0);
-  else
-    hipLaunchKernelGGL((k_patch_apply<__hip_bfloat16, float>), dim3(S->npatch), dim3(256), 0, s, S->npatch, S->PS,
-                       S->d_psize, S->d_pidx, S->d_pA16, (const float *)d_r, (float *)d_y, 1.0);
-  MHD_HIP(hipGetLastError());
-  return 0;
+  if (S->inner_prec == 0) return patch_as_on<double>(c, (const double *)d_r, (double *)d_y, 0, 1.0, true, s);
+  return patch_as_on<float>(c, (const float *)d_r, (float *)d_y, S->inner_prec, 1.0, true, s);
 }
 
 int mhd_smoother_info(mhd_ctx *c, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries) {

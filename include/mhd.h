@@ -208,7 +208,10 @@ int mhd_precond_setup(mhd_ctx *ctx, const mhd_solver_opts *opts);
 /* y += sum_P R_P^T A_P^-1 R_P r on the [u,j] block (device pointers, length
  * nfree[u] + nfree[j]; y is accumulated into).  The smoother of the [u,j]
  * block solve, in the precision of the last mhd_precond_setup: r and y are
- * double for precision 0 and float (the inner-solve vectors) for 1 and 2. */
+ * double for precision 0 and float (the inner-solve vectors) for 1 and 2.
+ * The patch products of a dof are summed in FP64 in a fixed order (the same
+ * result in every run) through a workspace of the context, so calls that
+ * overlap each other or a solve of the same context on another stream race. */
 int mhd_smoother_apply_d(mhd_ctx *ctx, const void *d_r, void *d_y, void *stream);
 /* Patch count, padded patch dimension, sum over patches of size^2. */
 int mhd_smoother_info(mhd_ctx *ctx, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries);
