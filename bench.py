@@ -11,7 +11,7 @@ walls.  Synthetic inputs only.
 A "step" is one Newton step: residual assembly + Jacobian assembly +
 preconditioner set-up + block-preconditioned FGMRES solve + update, from the
 zero initial guess (Newton tolerance 1e-10, linear tolerance 1e-11; the
-[u,j] block is solved inexactly by 3 flexible-GMRES steps preconditioned by a
+[u,j] block is solved inexactly by 4 flexible-GMRES steps preconditioned by a
 V-cycle over x/y semi-coarsened levels with vertex-star Schwarz smoothing and
 an exact coarsest solve).  `value` is the mixed-precision Newton-step time;
 the FP64 form is reported beside it, both with their relative solution error
@@ -41,8 +41,9 @@ FP64_PEAK_TFS = 78.6
 # HBM bytes per launch of the hot kernels at 64 x 64 x 8, measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) over
 # `bench.py --kernels-only`, each scaled by what the counter reads for a 1 GiB
-# device-to-device copy (FETCH_SIZE x 2, WRITE_SIZE x 1).
-C2_TRAFFIC = {"smoother_f32": 6.739e9, "smoother_f64": 1.3914e10, "spmv_f64": 5.991e9, "jacobian": 1.3641e10}
+# device-to-device copy (FETCH_SIZE x 2, WRITE_SIZE x 1).  The smoother
+# entries cover one application: k_patch_apply + k_patch_gather.
+C2_TRAFFIC = {"smoother_f32": 6.907e9, "smoother_f64": 1.4046e10, "spmv_f64": 5.991e9, "jacobian": 1.3641e10}
 # k_cell_jacobian at 64 x 64 x 8: SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 flops and
 # SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE per XCD x 1024 SIMDs), same runs.
 C2_JACOBIAN_MFMA = {"executed_flops": 5.979e10, "mfma_busy_frac": 0.231, "cell_kernel_ms": 3.341}
@@ -455,10 +456,11 @@ def main():
         "asm_residual_ms": round(t_res * 1e3, 4),
         "residual_cells_per_s": round(ncells / t_res, 1),
         # the dominant kernel of the mixed Newton step: the fine-level smoother
-        "roofline": roof(f"k_patch_apply<{'float' if args.mixed == 1 else 'bf16'}> vertex-star Schwarz, "
-                         f"fine level ({npatch} patches)", sm_bytes[args.mixed], t_sm[args.mixed],
+        "roofline": roof(f"k_patch_apply<{'float' if args.mixed == 1 else 'bf16'}> + k_patch_gather: one "
+                         f"vertex-star Schwarz application, fine level ({npatch} patches; bytes = the patch "
+                         f"inverses)", sm_bytes[args.mixed], t_sm[args.mixed],
                          "smoother_f32" if args.mixed == 1 else None),
-        "roofline_kernels": [roof("k_patch_apply<double> vertex-star Schwarz, fine level", sm_bytes[0], t_sm[0],
+        "roofline_kernels": [roof("k_patch_apply<double> + k_patch_gather, fine level", sm_bytes[0], t_sm[0],
                                   "smoother_f64"),
                              roof("k_spmv fp64 CSR", spmv_bytes, t_spmv, "spmv_f64"),
                              jac_roof],
