@@ -235,13 +235,9 @@ void mhd_destroy(mhd_ctx *c) {
   mhd_coarse_free(c);
   void *ptrs[] = {c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw,
                   c->d_rowptr, c->d_colind, c->d_rank, c->d_val, c->d_tperm, c->d_ref, c->d_fref, c->d_x, c->d_r,
-                  c->d_color_cells, c->d_crow, c->d_centry, c->d_ebuf};
+                  c->d_color_cells, c->d_crow, c->d_centry, c->d_ebuf, c->d_rbuf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  if (c->setup_stream) (void)hipStreamSynchronize(c->setup_stream);
-  if (c->setup_fork) (void)hipEventDestroy(c->setup_fork);
-  if (c->setup_join) (void)hipEventDestroy(c->setup_join);
-  if (c->setup_stream) (void)hipStreamDestroy(c->setup_stream);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -629,18 +629,32 @@ __global__ __launch_bounds__(WG, 2) void k_cell_jacobian(int ncells, const int32
   }
 }
 
+// Element residuals into rbuf[cell][NDOF] (free rows only); k_res_gather
+// then sums them per dof, so the residual does not depend on the order in
+// which the cells finish.
 __global__ __launch_bounds__(WG, 2) void k_cell_residual(int ncells, const double *__restrict__ xyz,
                                                       const int32_t *__restrict__ dofs, const int8_t *__restrict__ sign,
                                                       const double *__restrict__ dir, const double *__restrict__ x,
-                                                      const RefTables *__restrict__ T, double *__restrict__ r,
+                                                      const RefTables *__restrict__ T, double *__restrict__ rbuf,
                                                       MhdPhys ph, int mode) {
   __shared__ CellLDS S;
   for (int c = blockIdx.x; c < ncells; c += gridDim.x) {
     stage_cell(S, c, xyz, dofs, sign, dir, x, T, nullptr, ph, ph.zeta_u != 0.0);
     const int a = threadIdx.x;
-    if (a < NDOF && S.g[a] >= 0) atomicAdd(&r[S.g[a]], res_entry(S, T, ph, a, mode));
+    if (a < NDOF && S.g[a] >= 0) rbuf[(int64_t)c * NDOF + a] = res_entry(S, T, ph, a, mode);
     __syncthreads();
   }
+}
+
+// r[i] = sum of the element residual rows holding dof i, in increasing cell
+// order (the holder lists of the row assembly)
+__global__ void k_res_gather(int64_t n, const int32_t *__restrict__ crow, const int32_t *__restrict__ centry,
+                             const double *__restrict__ rbuf, double *__restrict__ r) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int32_t h = crow[i]; h < crow[i + 1]; ++h) s += rbuf[centry[h]];
+  r[i] = s;
 }
 
 // Per-cell integrals of the state (post-processing, src/Applications/hunt.jl:247-260):
@@ -1043,11 +1057,15 @@ int mhd_launch_jacobian(mhd_ctx *c, const double *d_x, double *d_val, hipStream_
 }
 
 int mhd_launch_residual_mode(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s, int mode) {
-  MHD_HIP(hipMemsetAsync(d_r, 0, sizeof(double) * c->n, s));
   if (c->ncells > 0 && mode != 2) {
+    if (!c->d_rbuf) MHD_HIP(hipMalloc(&c->d_rbuf, sizeof(double) * (size_t)c->ncells * NDOF));
     hipLaunchKernelGGL(k_cell_residual, dim3(c->ncells), dim3(WG), 0, s, c->ncells, c->d_xyz, c->d_dofs, c->d_sign,
-                       c->d_dir, d_x, c->d_ref, d_r, c->phys, mode);
+                       c->d_dir, d_x, c->d_ref, c->d_rbuf, c->phys, mode);
+    hipLaunchKernelGGL(k_res_gather, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, s, c->n, c->d_crow,
+                       c->d_centry, c->d_rbuf, d_r);
     MHD_HIP(hipGetLastError());
+  } else {
+    MHD_HIP(hipMemsetAsync(d_r, 0, sizeof(double) * c->n, s));
   }
   if (c->nfaces > 0 && mode != 1) {
     hipLaunchKernelGGL(k_face_residual, dim3(c->nfaces), dim3(64), 0, s, c->nfaces, c->d_xyz, c->d_dofs, c->d_sign,

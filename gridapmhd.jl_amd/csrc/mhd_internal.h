@@ -110,13 +110,11 @@ struct mhd_ctx {
   int32_t *d_centry = nullptr;        // [crow[n]], increasing cell order per row
   int32_t max_row = 0;                // longest CSR row
   double *d_ebuf = nullptr;           // [ncells][NENT] element buffer (allocated on first use)
+  double *d_rbuf = nullptr;           // [ncells][NDOF] element residuals (allocated on first use)
   // solver workspace (allocated lazily)
   struct Solver *solver = nullptr;
   // coarser level of the [u,j] block solve (owned), see mhd_attach_coarse
   struct CoarseLink *coarse = nullptr;
-  // the coarse levels' set-up runs on this stream beside the fine level's
-  hipStream_t setup_stream = nullptr;
-  hipEvent_t setup_fork = nullptr, setup_join = nullptr;
 };
 
 // Link from a level to the next coarser one: prolongation P (n1 x n1c CSR),

@@ -989,11 +989,10 @@ int coarse_alloc(mhd_ctx *c, int prec) {
 }
 
 // Per-Jacobian set-up of one level: Schur scalings, FP32 copy and patch
-// factorisation on c->stream, then the coarser levels on their own stream:
-// the u state is injected from `d_state` and the [u,j] block re-assembled
-// with the same kernels (rediscretisation).  This level's work is launched
-// first, so that with the coarse chain on a second stream the two overlap.
-int level_setup(mhd_ctx *c, int prec, const double *d_state) {
+// factorisation, then the coarser levels: the u state is injected from
+// `d_state` and the [u,j] block re-assembled with the same kernels
+// (rediscretisation).  Every level runs on the fine level's stream.
+int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
   Solver *S = c->solver;
   int rc;
   if ((rc = inner_alloc(c, prec))) return rc;
@@ -1058,7 +1057,7 @@ int level_setup(mhd_ctx *c, int prec, const double *d_state) {
     MHD_HIP(hipGetLastError());
     if (cc->solver) cc->solver->transposed = true;
   }
-  if ((rc = level_setup(cc, prec, cc->d_x))) {
+  if ((rc = precond_setup(cc, prec, cc->d_x))) {
     c->err = "coarse level: " + cc->err;
     return rc;
   }
@@ -1067,7 +1066,7 @@ int level_setup(mhd_ctx *c, int prec, const double *d_state) {
     // inverse as the LU solve A^T X = I (trsm-based, GEMM-rich): X = A^-T
     // column-major is A^-1 row-major, the layout of the row-dot product
     const int64_t m = L->n1c;
-    rocblas_handle rb = cc->solver->rb;  // bound to cs
+    rocblas_handle rb = cc->solver->rb;  // bound to cs, the stream of every level
     if (!L->d_Ainv) {
       MHD_HIP(hipMalloc(&L->d_Ainv, sizeof(double) * (size_t)(m * m)));
       MHD_HIP(hipMalloc(&L->d_LU, sizeof(double) * (size_t)(m * m)));
@@ -1095,39 +1094,6 @@ int level_setup(mhd_ctx *c, int prec, const double *d_state) {
       return -3;
     }
   }
-  return 0;
-}
-
-// Points every coarser level (its launches and its rocBLAS handle) at stream s.
-void chain_stream(mhd_ctx *c, hipStream_t s) {
-  for (mhd_ctx *q = c->coarse ? c->coarse->ctx : nullptr; q; q = q->coarse ? q->coarse->ctx : nullptr) {
-    q->stream = s;
-    if (q->solver && q->solver->rb) rocblas_set_stream(q->solver->rb, s);
-  }
-}
-
-// Set-up of the whole hierarchy: the coarse chain runs on a second stream,
-// forked after the work already queued on c->stream and joined back into it,
-// so the fine level's batched patch factorisation overlaps the coarse
-// levels' assembly, patch factorisations and dense coarsest LU.
-int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
-  if (!c->coarse) return level_setup(c, prec, d_state);
-  if (!c->setup_stream) {
-    MHD_HIP(hipStreamCreateWithFlags(&c->setup_stream, hipStreamNonBlocking));
-    MHD_HIP(hipEventCreateWithFlags(&c->setup_fork, hipEventDisableTiming));
-    MHD_HIP(hipEventCreateWithFlags(&c->setup_join, hipEventDisableTiming));
-  }
-  MHD_HIP(hipEventRecord(c->setup_fork, c->stream));
-  MHD_HIP(hipStreamWaitEvent(c->setup_stream, c->setup_fork, 0));
-  chain_stream(c, c->setup_stream);
-  const int rc = level_setup(c, prec, d_state);
-  // join also after an error, so that the chain is back on c->stream
-  const hipError_t e1 = hipEventRecord(c->setup_join, c->setup_stream);
-  const hipError_t e2 = hipStreamWaitEvent(c->stream, c->setup_join, 0);
-  chain_stream(c, c->stream);
-  if (rc) return rc;
-  MHD_HIP(e1);
-  MHD_HIP(e2);
   return 0;
 }
 

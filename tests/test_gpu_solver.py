@@ -167,3 +167,30 @@ def test_feature_cases_match_direct(name, precision, orc_mod):
             assert err["all"] < bar, err
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_solves_repeat_bitwise(precision):
+    """Two Newton solves of the same problem in one process give the same bits:
+    the residual, the Jacobian and the smoother sum their contributions in
+    fixed orders (no atomics on the default paths)."""
+    import gridapmhd_amd as g
+    _say(f"[solver] repeatability precision={precision}")
+    sp, phys = cases.duct_thin_wall_case((16, 16, 3), Ha=30.0, Re=10.0, cw=0.02, convection="newton",
+                                         style="badia2024")
+    ctx = g._lib.context_with_levels(sp, dict(phys, zeta_u=1.0, zeta_j=1.0), levels=3, coarse_iters=0,
+                                     coarse_step="full")
+    try:
+        x0 = np.random.default_rng(11).standard_normal(sp.ndofs) * 0.1
+        r1, r2 = ctx.residual(x0), ctx.residual(x0)
+        assert np.array_equal(r1, r2)
+        o = _opts(g, precision)
+        o.inner_iters = 4
+        xa, ia, rca = ctx.newton_solve(np.zeros(sp.ndofs), o)
+        xb, ib, rcb = ctx.newton_solve(np.zeros(sp.ndofs), o)
+        _say(f"  krylov {ia.krylov_iters} / {ib.krylov_iters}  |R| {ia.residual!r} / {ib.residual!r}")
+        assert rca == 0 and rcb == 0
+        assert ia.krylov_iters == ib.krylov_iters
+        assert np.array_equal(xa, xb)
+    finally:
+        ctx.close()
