@@ -246,7 +246,7 @@ def main():
     ap.add_argument("--inner", type=int, default=4, help="flexible-GMRES steps of the inexact [u,j] solve")
     ap.add_argument("--levels", default="auto", help="levels of the [u,j] V-cycle (1 = Schwarz only; auto: 5 at C2)")
     ap.add_argument("--coarse-iters", type=int, default=0, help="0: exact coarsest solve")
-    ap.add_argument("--weight", type=float, default=0.2, help="Schwarz smoother damping in the V-cycle")
+    ap.add_argument("--weight", type=float, default=0.175, help="Schwarz smoother damping in the V-cycle")
     ap.add_argument("--coarse-step", default="full", choices=["full", "minres"],
                     help="coarse correction step (full: fewest iterations on this uniform duct)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")

@@ -242,7 +242,7 @@ class Context:
             setattr(o, k, v)
         return o
 
-    def attach_coarse(self, coarse: "Context", P, inj_u, coarse_iters: int = 0, smoother_weight: float = 0.2,
+    def attach_coarse(self, coarse: "Context", P, inj_u, coarse_iters: int = 0, smoother_weight: float = 0.175,
                       coarse_step: str = "minres"):
         """Hand `coarse` (and the prolongation P, scipy CSR) to this context.
         coarse_step: "full" or "minres" (minimal-residual scaling of the
@@ -311,7 +311,7 @@ class Context:
         return x, info, rc
 
 
-def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.2, coarse_step="minres",
+def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.175, coarse_step="minres",
                         device=0) -> Context:
     """Fine context plus `levels - 1` semi-coarsened levels attached for the
     [u,j] V-cycle (badia2024 numbering, nested x/y coarsening, the same

@@ -132,7 +132,7 @@ struct CoarseLink {
   double *d_Ainv = nullptr;        // [n1c][n1c]
   double *d_LU = nullptr;          // [n1c][n1c] LU factors (set-up scratch)
   int32_t *d_piv = nullptr, *d_info = nullptr;
-  double omega = 0.2;              // damping of the Schwarz smoother inside the V-cycle
+  double omega = 0.175;            // damping of the Schwarz smoother inside the V-cycle
   int step_mode = 1;               // coarse correction step: 0 full, 1 minimal residual
   void *rr = nullptr;              // [3 (n1 + 1)] fine residual, A11 yc, yc (T)
   double *d_ab = nullptr;          // the two dots of the minimal-residual coarse step

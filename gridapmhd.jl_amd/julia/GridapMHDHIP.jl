@@ -72,7 +72,7 @@ function GridapMHD.default_solver_params(::Val{:hip})
     :inner_iters => 4, :restart => 200,
     :levels => :auto,              # [u,j] V-cycle levels (x/y semi-coarsening), see attach_levels!
     :coarse_iters => 0,            # 0: exact coarsest solve
-    :smoother_weight => 0.2,
+    :smoother_weight => 0.175,
     :coarse_step => :minres,       # :minres | :full
   )
 end
@@ -117,13 +117,13 @@ function Gridap.Algebra.solve!(x::AbstractVector, ns::HIPNewtonSolver, op::Grida
 end
 
 """
-    attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=0, ω=0.2, step=:minres)
+    attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=0, ω=0.175, step=:minres)
 
 Chains coarser contexts (built from `hunt_generate_mesh_hierarchy`-style
 x,y-semi-coarsened models) onto `ctx` through `mhd_attach_coarse`; each
 prolongation is a `SparseMatrixCSR` fine x coarse on the [u; j] dofs.
 """
-function attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=0, ω=0.2, step=:minres)
+function attach_levels!(ctx, coarse_ctxs, prolongations, injections; coarse_iters=0, ω=0.175, step=:minres)
   for lev in length(coarse_ctxs):-1:1
     fine = lev == 1 ? ctx : coarse_ctxs[lev-1]
     P = prolongations[lev]

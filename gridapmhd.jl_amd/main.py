@@ -83,7 +83,7 @@ def _fe_operator(sp, params) -> _lib.Context:
     s = params["solver"]
     return _lib.context_with_levels(sp, fluid_physics(params["fluid"]), levels=s.get("levels", 1),
                                     coarse_iters=int(s.get("coarse_iters", 0)),
-                                    smoother_weight=float(s.get("smoother_weight", 0.2)),
+                                    smoother_weight=float(s.get("smoother_weight", 0.175)),
                                     coarse_step=s.get("coarse_step", "minres"))
 
 

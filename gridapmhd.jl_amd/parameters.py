@@ -80,7 +80,7 @@ def default_solver_params(name: str) -> Dict[str, Any]:
             "inner_iters": 4,     # flexible-GMRES steps of the inexact [u,j] solve
             "levels": "auto",     # [u,j] V-cycle levels (x/y semi-coarsening); 1 = Schwarz only
             "coarse_iters": 0,    # coarsest-level solve: 0 = exact (dense inverse), k = k Schwarz-GMRES steps
-            "smoother_weight": 0.2,
+            "smoother_weight": 0.175,
             "coarse_step": "minres",  # coarse correction step: "minres" (robust) | "full"
             "block_solvers": ["hip_vertex_schwarz", "mass_inverse", "mass_inverse"],
         }

@@ -188,7 +188,7 @@ int mhd_linear_solve(mhd_ctx *ctx, const double *x_state, const double *rhs, dou
  * ownership of coarse (destroyed with ctx; do not destroy it separately).  The
  * [u,j] smoother of mhd_newton_solve then becomes a V-cycle whose vertex-star
  * Schwarz pre/post-smoothing is damped by smoother_weight (0, 1] (the patches
- * overlap up to 8-fold; 0.2 is a good value).  On the coarsest level
+ * overlap up to 8-fold; 0.175 is a good value, 0.25 can diverge).  On the coarsest level
  * coarse_iters Schwarz-GMRES steps are taken, or, with coarse_iters = 0, the
  * level is solved exactly with a dense FP64 inverse (at most 32768 [u,j]
  * dofs there).  coarse_step 0 adds the coarse correction with a full step;
