@@ -11,7 +11,7 @@ walls.  Synthetic inputs only.
 A "step" is one Newton step: residual assembly + Jacobian assembly +
 preconditioner set-up + block-preconditioned FGMRES solve + update, from the
 zero initial guess (Newton tolerance 1e-10, linear tolerance 1e-11; the
-[u,j] block is solved inexactly by 4 flexible-GMRES steps preconditioned by a
+[u,j] block is solved inexactly by 3 flexible-GMRES steps preconditioned by a
 V-cycle over x/y semi-coarsened levels with vertex-star Schwarz smoothing and
 an exact coarsest solve).  `value` is the mixed-precision Newton-step time;
 the FP64 form is reported beside it, both with their relative solution error
@@ -243,7 +243,7 @@ def main():
     ap.add_argument("--asm-reps", type=int, default=10)
     ap.add_argument("--mixed", type=int, default=1, choices=[1, 2], help="mixed mode: 1 FP32 / 2 BF16 patch blocks")
     ap.add_argument("--restart", type=int, default=200)
-    ap.add_argument("--inner", type=int, default=4, help="flexible-GMRES steps of the inexact [u,j] solve")
+    ap.add_argument("--inner", type=int, default=3, help="flexible-GMRES steps of the inexact [u,j] solve")
     ap.add_argument("--levels", default="auto", help="levels of the [u,j] V-cycle (1 = Schwarz only; auto: 5 at C2)")
     ap.add_argument("--coarse-iters", type=int, default=0, help="0: exact coarsest solve")
     ap.add_argument("--weight", type=float, default=0.175, help="Schwarz smoother damping in the V-cycle")

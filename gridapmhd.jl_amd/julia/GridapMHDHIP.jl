@@ -69,7 +69,7 @@ function GridapMHD.default_solver_params(::Val{:hip})
     :solver_postpro => ((cache, info) -> hip_postpro(cache, info)),
     :niter => 20, :niter_ls => 400, :rtol => 1e-6, :atol => 1e-8,
     :precision => :fp64,           # :fp64 | :mixed | :mixed_bf16
-    :inner_iters => 4, :restart => 200,
+    :inner_iters => 3, :restart => 200,
     :levels => :auto,              # [u,j] V-cycle levels (x/y semi-coarsening), see attach_levels!
     :coarse_iters => 0,            # 0: exact coarsest solve
     :smoother_weight => 0.175,

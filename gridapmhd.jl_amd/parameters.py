@@ -77,7 +77,7 @@ def default_solver_params(name: str) -> Dict[str, Any]:
             "atol": 1e-8,
             "precision": "fp64",  # "fp64" | "mixed" | "mixed_bf16"
             "restart": 200,
-            "inner_iters": 4,     # flexible-GMRES steps of the inexact [u,j] solve
+            "inner_iters": 3,     # flexible-GMRES steps of the inexact [u,j] solve
             "levels": "auto",     # [u,j] V-cycle levels (x/y semi-coarsening); 1 = Schwarz only
             "coarse_iters": 0,    # coarsest-level solve: 0 = exact (dense inverse), k = k Schwarz-GMRES steps
             "smoother_weight": 0.175,
