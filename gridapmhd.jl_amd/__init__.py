@@ -7,7 +7,7 @@ Public surface (mirrors the reference's entry points):
   meshers.hunt_generate_base_mesh, fespaces.setup_fe_spaces
   _lib.Context                                      ctypes binding of include/mhd.h
 """
-from . import meshers, fespaces, _lib, parameters, multigrid  # noqa: F401
+from . import meshers, fespaces, _lib, parameters, multigrid, analytic, postprocess  # noqa: F401
 from .parameters import add_default_params, default_solver_params  # noqa: F401
 from .main import main, MultiFieldFEFunction  # noqa: F401
-from .applications import hunt, thin_wall_duct  # noqa: F401
+from .applications import hunt, thin_wall_duct, fully_developed  # noqa: F401

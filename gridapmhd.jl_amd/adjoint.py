@@ -23,10 +23,14 @@ _W = (5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0)
 
 
 def inlet_duct(nc, Ha, Re, cw, tau=100.0, U=1.0, domain=(-1.0, 1.0, -1.0, 1.0, 0.0, 0.3), convection="newton",
-               zeta=1.0, style="badia2024"):
-    """Spaces + physics of the developing thin-walled duct; inlet u = (0, 0, U (1-x^2)(1-y^2))."""
+               zeta=1.0, style="badia2024", jitter=0.0, seed=20261202):
+    """Spaces + physics of the developing thin-walled duct; inlet u = (0, 0, U (1-x^2)(1-y^2)).
+    `jitter` > 0 moves the interior nodes by U[-jitter h, jitter h]^3 (seeded):
+    non-affine cells, with the inlet and outlet planes left flat."""
     mesh = meshers.cartesian_model(domain, nc, (False, False, False))
     meshers.hunt_add_tags(mesh)
+    if jitter:
+        meshers.jitter_nodes(mesh, jitter, seed)
     sp = fespaces.setup_fe_spaces(mesh, ["walls", "inlet"], "ends",
                                   [dict(domain="walls", cw=cw, tau=tau, jw=0.0)], style=style)
     prof = lambda x: np.stack([0 * x[:, 0], 0 * x[:, 0], U * (1 - x[:, 0] ** 2) * (1 - x[:, 1] ** 2)], 1)

@@ -3,8 +3,9 @@
 ``hunt`` mirrors ``_hunt`` (src/Applications/hunt.jl:41-304) for the
 sequential H1-HDiv case: non-dimensionalisation (:cfd or :mhd), the
 Cartesian duct mesh with the reference's tags, noslip walls, insulating side
-walls, and the post-processed norms uh_l2, uh_h1, jh_l2 (with the rescaling
-u = u0 ubar, j = sigma u0 B0 jbar).  ``thin_wall_duct`` is the fully developed
+walls, the post-processed norms uh_l2, uh_h1, jh_l2 (with the rescaling
+u = u0 ubar, j = sigma u0 B0 jbar) and the errors eu_h1, eu_l2, ej_l2 against
+the analytic solution (``analytic.analytical_hunt_u/j``).  ``thin_wall_duct`` is the fully developed
 duct with thin conducting walls on all four walls, the pattern of
 src/Applications/FullyDevelopedFlow.jl:96-194 used by the benchmarks.
 """
@@ -14,14 +15,14 @@ import math
 
 import numpy as np
 
-from . import meshers
+from . import analytic, meshers, postprocess
 from .main import main
 
 
 def hunt(nc=(4, 4), nz=3, ν=1.0, ρ=1.0, σ=1.0, B=(0.0, 10.0, 0.0), f=(0.0, 0.0, 1.0), ζᵤ=0.0, ζⱼ=0.0, L=1.0,
          u0=1.0, B0=None, order=2, order_j=None, formulation="cfd", convection="none", BL_adapted=True,
          kmap_x=1, kmap_y=1, solver="hip", initial_value="zero", domain=(-1.0, 1.0, -1.0, 1.0, 0.0, 0.1),
-         debug=False, solve=True, res_assemble=False, jac_assemble=False):
+         debug=False, solve=True, res_assemble=False, jac_assemble=False, nsums=10, errors=True):
     B0 = float(np.linalg.norm(B)) if B0 is None else B0
     order_j = order if order_j is None else order_j
     info = {}
@@ -56,6 +57,16 @@ def hunt(nc=(4, 4), nz=3, ν=1.0, ρ=1.0, σ=1.0, B=(0.0, 10.0, 0.0), f=(0.0, 0.
     info["uh_h1"] = u0 * math.sqrt(ci[0] + ci[1])
     info["jh_l2"] = σ * u0 * B0 * math.sqrt(ci[2])
     info["divj_l2"] = σ * u0 * B0 * math.sqrt(ci[3])
+    if errors and solve:
+        # eu = u - uh, ej = j - jh against the analytic Hunt solution with nsums and
+        # 2 nsums modes (src/Applications/hunt.jl:237-254); L = 1 here, so the
+        # physical and the computational meshes coincide.
+        mu, grad_pz = ρ * ν, -float(f[2]) / ρ
+        for suffix, ns in (("", nsums), ("_ref", 2 * nsums)):
+            e = postprocess.hunt_errors(xh.spaces, xh.free_values, L, mu, σ, grad_pz, Ha, ns, u_scale=u0,
+                                        j_scale=σ * u0 * B0)
+            info["eu_h1" + suffix], info["eu_l2" + suffix], info["ej_l2" + suffix] = e["eu_h1"], e["eu_l2"], e["ej_l2"]
+        info["nsums"] = nsums
     sp = xh.spaces
     info.update(ncells=sp.ncells, ndofs=sp.ndofs, ndofs_u=sp.nfree["u"], ndofs_p=sp.nfree["p"],
                 ndofs_j=sp.nfree["j"], ndofs_φ=sp.nfree["phi"], Re=Re, Ha=Ha)
@@ -76,3 +87,69 @@ def thin_wall_duct(nc=(64, 64, 8), Ha=100.0, Re=10.0, cw=0.01, τ=100.0, domain=
         "solver": s,
     }
     return main(params)
+
+
+def fully_developed_problem(nc=(3, 3), Ha=10.0, dir_B=(0.0, 1.0, 0.0), b=1.0, cw_Ha=0.0, cw_s=0.0, τ_Ha=100.0,
+                            τ_s=100.0, strech_fine=False, ζ=1.0, nz=3):
+    """Mesh, boundary conditions and :mhd-scaled fluid parameters of
+    ``_FullyDeveloped`` (src/Applications/FullyDevelopedFlow.jl:44-194):
+    duct (-b, b) x (-1, 1) x (0, 0.1), periodic in z, Smolentsev stretching
+    towards the walls, no-slip everywhere, Hartmann walls (y = +-1) and side
+    walls (x = +-b) either insulating (cw = 0: j.n = 0) or thin conducting
+    walls with conductance ratio cw and penalty tau.  Returns (params, model)."""
+    Re = 1.0
+    N = Ha * Ha / Re
+    dB = np.asarray(dir_B, float) / np.linalg.norm(dir_B)
+    s_Ha = math.sqrt(Ha / (Ha - 1.0))
+    s_side = math.sqrt(math.sqrt(Ha) / (math.sqrt(Ha) - 1.0))
+    fx = (s_Ha, s_Ha) if strech_fine else (s_side, s_Ha)
+
+    def map1(x):
+        y = meshers.strech_mhd(x, (0.0, -b, 0.0, -1.0), fx, (0, 1))
+        return meshers.strech_mhd(y, (0.0, b, 0.0, 1.0), fx, (0, 1))
+
+    model = meshers.cartesian_model((-b, b, -1.0, 1.0, 0.0, 0.1), (nc[0], nc[1], nz), (False, False, True), map1)
+    meshers.hunt_add_tags(model)
+    insulated, thin = [], []
+    for tag, cw, tau in (("Ha_walls", cw_Ha, τ_Ha), ("side_walls", cw_s, τ_s)):
+        if cw == 0.0:
+            insulated.append(tag)
+        else:
+            thin.append({"domain": tag, "cw": cw, "τ": tau})
+    params = {
+        "model": model,
+        "fluid": {"domain": None, "α": 1.0 / N, "β": 1.0 / Ha ** 2, "γ": 1.0, "f": (0.0, 0.0, 1.0), "B": tuple(dB),
+                  "ζᵤ": ζ, "ζⱼ": ζ},
+        "bcs": {"u": {"tags": ["Ha_walls", "side_walls"]}, "j": {"tags": insulated}, "thin_wall": thin},
+    }
+    return params, model
+
+
+def fully_developed(nc=(3, 3), Ha=10.0, dir_B=(0.0, 1.0, 0.0), b=1.0, cw_Ha=0.0, cw_s=0.0, τ_Ha=100.0, τ_s=100.0,
+                    nsums=10, strech_fine=False, ζ=1.0, solver="hip", **solver_kw):
+    """``_FullyDeveloped`` (src/Applications/FullyDevelopedFlow.jl:44-270) on
+    libmhd: solves the duct and post-processes the mean velocity uh_0, the
+    pressure-gradient coefficient kp = 1/uh_0 against the correlation
+    (kp_shercliff_cartesian for insulating walls, kp_tillac otherwise) and,
+    with insulating side walls, the relative L2 distance of u_h to the
+    general Hunt solution with conducting Hartmann walls
+    (analytical_general_hunt_u, d_b = cw_Ha)."""
+    params, model = fully_developed_problem(nc, Ha, dir_B, b, cw_Ha, cw_s, τ_Ha, τ_s, strech_fine, ζ)
+    s = {"solver": solver}
+    s.update(solver_kw)
+    params["solver"] = s
+    info = {}
+    xh, full, out = main(params, output=info)
+    sp = xh.spaces
+    uh_0 = postprocess.mean_uz(sp, xh.free_values)
+    kp = 1.0 / uh_0
+    if cw_s == 0.0 and cw_Ha == 0.0:
+        kp_a = analytic.kp_shercliff_cartesian(b, Ha)
+    else:
+        kp_a = analytic.kp_tillac(b, Ha, cw_s, cw_Ha)
+    info.update(uh_0=uh_0, kp=kp, kp_a=kp_a, dev_kp=100.0 * abs(kp_a - kp) / max(kp_a, kp), nc=nc, Ha=Ha,
+                ncells=sp.ncells, ndofs=sp.ndofs, ndofs_u=sp.nfree["u"], ndofs_p=sp.nfree["p"],
+                ndofs_j=sp.nfree["j"], ndofs_φ=sp.nfree["phi"])
+    if cw_s == 0.0:
+        info["e_u_l2_rel"] = postprocess.general_hunt_error(sp, xh.free_values, b, cw_Ha, -1.0, Ha, nsums)
+    return info, xh

@@ -114,3 +114,45 @@ def field_errors(sp, x, xref):
     xb[ps] -= xb[ps].mean()
     out["all"] = np.linalg.norm(xa - xb) / np.linalg.norm(xb)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Oracle fixtures: cases whose oracle direct solves take minutes (8 x 8 x 8
+# cells, ~30K dofs; SuperLU factorisations of ~90 s each) are solved once by
+# tests/golden/make_oracle_fixtures.py and stored under tests/golden/.  The
+# CPU suite re-checks each stored vector against the oracle (residual, and
+# the transposed system for adjoints), which takes a second.
+
+FIXTURE_DIR = os.path.join(ROOT, "tests", "golden")
+
+# duct 8 x 8 x 8 with the physics of the 64 x 64 x 8 benchmark duct
+C2_PARAMS = dict(nc=(8, 8, 8), Ha=100.0, Re=10.0, cw=0.01, convection="newton")
+# developing duct (inlet profile, outlet, thin walls): pressure-drop adjoint
+ADJOINT_CASES = {
+    "c4_adjoint_8x8x8": dict(nc=(8, 8, 8), Ha=100.0, Re=10.0, cw=0.01, jitter=0.0),
+    "c5_adjoint_jitter_cw0.001": dict(nc=(6, 6, 6), Ha=500.0, Re=50.0, cw=0.001, jitter=0.2),
+    "c5_adjoint_jitter_cw0.01": dict(nc=(6, 6, 6), Ha=500.0, Re=50.0, cw=0.01, jitter=0.2),
+    "c5_adjoint_jitter_cw0.1": dict(nc=(6, 6, 6), Ha=500.0, Re=50.0, cw=0.1, jitter=0.2),
+}
+
+
+def c2_params_case(style="badia2024"):
+    return duct_thin_wall_case(style=style, **C2_PARAMS)
+
+
+def adjoint_case(name, zeta=None):
+    """Spaces + physics of a developing-duct adjoint case.  The oracle solves
+    the plain method (zeta = 0) on affine cells; on jittered cells the
+    div-div terms do not vanish at the discrete solution, so the oracle then
+    solves the same augmented problem as the GPU (zeta = 1)."""
+    from gridapmhd_amd import adjoint as adj
+    kw = dict(ADJOINT_CASES[name])
+    jitter = kw.pop("jitter")
+    if zeta is None:
+        zeta = 1.0 if jitter else 0.0
+    return adj.inlet_duct(kw["nc"], kw["Ha"], kw["Re"], kw["cw"], zeta=zeta, jitter=jitter)
+
+
+def load_fixture(name):
+    with np.load(os.path.join(FIXTURE_DIR, f"oracle_{name}.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}

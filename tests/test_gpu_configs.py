@@ -1,0 +1,131 @@
+"""GPU parity at the benchmark configurations' parameters and sizes.
+
+* The 64 x 64 x 8 benchmark duct's physics and solver options (Ha = 100,
+  Re = 10, cw = 0.01, Newton, V-cycle with damping 0.175, full coarse step,
+  3 inner steps, exact coarsest solve) on 8 x 8 x 8 cells, against the
+  oracle's Newton + direct solve (fixture tests/golden/oracle_c2_params_8x8x8.npz,
+  re-checked against the oracle by tests/test_fixtures.py).  Also from a
+  random start, so that Newton takes several steps with the convective
+  Jacobian terms active.
+* The Hunt duct through hunt() -> main(params) at 96 x 96 x 3 (Ha = 50,
+  1,651,986 dofs) and 64 x 64 x 3 (Ha = 500), against the reference's own
+  published solution norms (tests/golden/gadi_hunt.json).
+* The full 64 x 64 x 8 duct (1,957,936 dofs): both precisions converge, the
+  mixed solution is within 1e-8 of the FP64 one, the oracle's residual of
+  the GPU solution is at the FP64 floor, div j_h = 0, and a repeated mixed
+  solve gives the same bits.
+
+Bars: FP64 1e-10, mixed 1e-8 relative to the oracle solution (pressure and
+potential modulo their constants); golden norms 1e-9.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "gadi_hunt.json")))
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _bench_opts(ctx, precision, rtol=1e-12, ls_rtol=1e-12, max_newton=8):
+    """The 64 x 64 x 8 benchmark's Krylov options (restart 200, 3 inner steps)."""
+    return ctx.default_opts(precision=precision, rtol=rtol, atol=1e-16, ls_rtol=ls_rtol, max_krylov=3000,
+                            restart=200, inner_iters=3, max_newton=max_newton)
+
+
+@pytest.mark.parametrize("start", ["zero", "random"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_c2_parameters_match_oracle(precision, start):
+    import gridapmhd_amd as g
+    _say(f"[configs] 64x64x8 physics on 8x8x8, bench solver options, precision={precision} start={start}")
+    fx = cases.load_fixture("c2_params_8x8x8")
+    sp, phys = cases.c2_params_case()
+    assert sp.ndofs == fx["x"].size
+    ctx = g._lib.context_with_levels(sp, dict(phys, zeta_u=1.0, zeta_j=1.0), levels=2, coarse_iters=0,
+                                     smoother_weight=0.175, coarse_step="full")
+    try:
+        x0 = np.zeros(sp.ndofs)
+        o = _bench_opts(ctx, precision, max_newton=20)
+        if start == "random":
+            x0 = 0.5 * np.random.default_rng(2024).standard_normal(sp.ndofs)
+            # the random start's residual is ~1e3 times the one at zero: stop on
+            # the same absolute residual as the solve from zero
+            o.rtol, o.atol = 1e-30, 1e-12 * np.linalg.norm(ctx.residual(np.zeros(sp.ndofs)))
+        x, info, rc = ctx.newton_solve(x0, o)
+        err = cases.field_errors(sp, x, fx["x"])
+        _say(f"  newton={info.newton_iters} krylov={info.krylov_iters} rc={rc} "
+             f"|R|/|R0|={info.residual / info.residual0:.1e} " + " ".join(f"{k}={v:.2e}" for k, v in err.items()))
+        assert rc == 0 and info.converged
+        if start == "random":
+            assert info.newton_iters >= 3          # the convective Jacobian terms were exercised
+        assert err["all"] < (1e-10 if precision == 0 else 1e-8), err
+    finally:
+        ctx.close()
+
+
+HUNT_AT_SCALE = [(96, 50.0), (64, 500.0)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("cx,Ha", HUNT_AT_SCALE)
+def test_hunt_at_scale_matches_reference_norms(cx, Ha, precision):
+    import gridapmhd_amd as g
+    ref = [r for r in GOLD if r["cx"] == cx and r["Ha"] == Ha and r["kmap"] == "1"][0]
+    _say(f"[configs] hunt cx={cx} Ha={Ha:g} precision={precision} (reference {ref['source']})")
+    solver = {"solver": "hip", "rtol": 1e-11, "atol": 1e-16, "niter_ls": 3000, "precision": precision,
+              "restart": 200}
+    info, xh = g.hunt(nc=(cx, cx), B=(0.0, Ha, 0.0), ζᵤ=1.0, ζⱼ=1.0, BL_adapted=False, solver=solver)
+    _say(f"  ndofs={info['ndofs']} converged={info['converged']} ls_iters={info.get('ls_iters')}")
+    assert info["ndofs"] == ref["ndofs"] and info["converged"]
+    for k in ("uh_l2", "uh_h1", "jh_l2"):
+        rel = abs(info[k] / ref[k] - 1.0)
+        _say(f"  {k}: {info[k]:.15g} vs {ref[k]:.15g} rel {rel:.2e}")
+        assert rel < 1e-9, (k, rel)
+    assert info["divj_l2"] < 1e-9 * info["jh_l2"]
+
+
+def test_c2_full_size_properties(orc_mod):
+    import gridapmhd_amd as g
+    _say("[configs] 64x64x8 duct, Ha=100 Re=10 cw=0.01: FP64 and mixed Newton solves")
+    sp, phys = cases.duct_thin_wall_case((64, 64, 8), Ha=100.0, Re=10.0, cw=0.01, convection="newton",
+                                         style="badia2024")
+    gphys = dict(phys, zeta_u=1.0, zeta_j=1.0)
+    ctx = g._lib.context_with_levels(sp, gphys, levels="auto", coarse_iters=0, smoother_weight=0.175,
+                                     coarse_step="full")
+    try:
+        x0 = np.zeros(sp.ndofs)
+        xd, i64, rc64 = ctx.newton_solve(x0, _bench_opts(ctx, 0, rtol=1e-10, ls_rtol=1e-11))
+        _say(f"  fp64: newton={i64.newton_iters} krylov={i64.krylov_iters} rc={rc64} |R|/|R0|="
+             f"{i64.residual / i64.residual0:.1e}")
+        xm, im, rcm = ctx.newton_solve(x0, _bench_opts(ctx, 1, rtol=1e-10, ls_rtol=1e-11))
+        _say(f"  mixed: newton={im.newton_iters} krylov={im.krylov_iters} rc={rcm}")
+        xm2, im2, _ = ctx.newton_solve(x0, _bench_opts(ctx, 1, rtol=1e-10, ls_rtol=1e-11))
+        _say(f"  mixed again: krylov={im2.krylov_iters} bitwise={np.array_equal(xm, xm2)}")
+        ci = ctx.cell_integrals(xd).sum(axis=0)
+    finally:
+        ctx.close()
+    assert rc64 == 0 and rcm == 0 and i64.converged and im.converged
+    err = cases.field_errors(sp, xm, xd)
+    _say("  mixed vs fp64 " + " ".join(f"{k}={v:.2e}" for k, v in err.items()))
+    assert err["all"] < 1e-8, err
+    assert np.array_equal(xm, xm2) and im.krylov_iters == im2.krylov_iters
+    divj = np.sqrt(ci[3]) / np.sqrt(ci[2])
+    _say(f"  |div j_h| / |j_h| = {divj:.2e}")
+    assert divj < 1e-9
+    # the oracle's own residual (the plain method, zeta = 0) of the GPU solutions
+    orc = orc_mod.Oracle(cases.problem_dict(sp, phys))
+    r0 = np.linalg.norm(orc.residual(x0))
+    for name, xs in (("fp64", xd), ("mixed", xm)):
+        rel = np.linalg.norm(orc.residual(xs)) / r0
+        _say(f"  oracle |R(x_{name})| / |R(0)| = {rel:.2e}")
+        assert rel < 1e-9, (name, rel)

@@ -33,3 +33,44 @@ def test_thin_wall_duct_main_converges():
                                      precision="mixed", rtol=1e-10, atol=1e-14, niter_ls=1000)
     assert out["converged"] and out["nls_iters"] >= 1 and out["ls_iters"] > 0
     assert set(out["timings"]) >= {"fe_spaces", "solver_setup", "solve"}
+
+
+def test_hunt_analytic_errors_match_reference():
+    """eu_h1 / ej_l2 of the GPU Hunt solution against the analytic series equal
+    the reference's published values (cx = 10, Ha = 50, nsums = 500)."""
+    import gridapmhd_amd as g
+    print("[main] hunt cx=10 Ha=50 analytic errors", flush=True)
+    ref = [r for r in GOLD if r["cx"] == 10 and r["Ha"] == 50.0][0]
+    solver = {"solver": "hip", "rtol": 1e-12, "atol": 1e-16, "niter_ls": 2000, "precision": "fp64"}
+    info, _ = g.hunt(nc=(10, 10), B=(0.0, 50.0, 0.0), ζᵤ=1.0, ζⱼ=1.0, BL_adapted=False, solver=solver,
+                     nsums=ref["nsums"])
+    for k in ("eu_h1", "ej_l2"):
+        rel = abs(info[k] / ref[k] - 1.0)
+        print(f"  {k}: {info[k]:.15g} vs {ref[k]:.15g} rel {rel:.2e}", flush=True)
+        assert rel < 1e-9, (k, rel)
+
+
+def test_fully_developed_thin_wall_converges_to_general_hunt():
+    """FullyDeveloped duct (Ha = 10, thin conducting Hartmann walls cw = 0.1,
+    tau = 1e4, insulating side walls) through main(params) on the GPU: the
+    distance to analytical_general_hunt_u shrinks with h on 8, 16, 32 cells
+    per side, and the penalty limit is reached as tau grows."""
+    import gridapmhd_amd as g
+    errs = {}
+    for n in (8, 16, 32):
+        info, _ = g.fully_developed(nc=(n, n), Ha=10.0, cw_Ha=0.1, τ_Ha=1e4, nsums=50, rtol=1e-11, atol=1e-16,
+                                    niter_ls=2000, precision="fp64")
+        errs[n] = info["e_u_l2_rel"]
+        print(f"[main] fully developed {n}x{n}x3 cw_Ha=0.1 tau=1e4: e_u={errs[n]:.3e} kp={info['kp']:.5f} "
+              f"kp_tillac={info['kp_a']:.5f} converged={info['converged']}", flush=True)
+        assert info["converged"]
+    assert errs[16] < 0.6 * errs[8] and errs[32] < 0.6 * errs[16] and errs[32] < 3e-3, errs
+    taus = {}
+    for tau in (1e2, 1e4, 1e6):
+        info, _ = g.fully_developed(nc=(16, 16), Ha=10.0, cw_Ha=0.1, τ_Ha=tau, nsums=50, rtol=1e-11, atol=1e-16,
+                                    niter_ls=2000, precision="fp64")
+        taus[tau] = info["uh_0"]
+        print(f"[main] fully developed 16x16x3 tau={tau:g}: uh_0={info['uh_0']:.10f} e_u={info['e_u_l2_rel']:.3e}",
+              flush=True)
+    # the penalty converges: the 1e4 -> 1e6 change is far below the 1e2 -> 1e4 one
+    assert abs(taus[1e6] - taus[1e4]) < 0.05 * abs(taus[1e4] - taus[1e2])
