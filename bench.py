@@ -222,6 +222,7 @@ def kernels_only(ctx, sp, x, val, r, stream, sh, times):
     n1 = sp.nfree["u"] + sp.nfree["j"]
     out = {k: round(v * 1e3, 4) for k, v in times.items()}
     out["copy_1GiB"] = round(t_copy * 1e3, 4)
+    ctx.jacobian_d(x.data_ptr(), 0, sh)          # into the context: the Jacobian the set-up factors
     for prec, dt in ((0, torch.float64), (1, torch.float32)):
         ctx.precond_setup(ctx.default_opts(precision=prec))
         rr = torch.rand(n1, dtype=dt, device=dev)
@@ -382,7 +383,7 @@ def main():
         log(f"{label}: {results[label]}")
 
     # ---- smoother kernel (vertex-star additive Schwarz), FP64 and FP32 blocks --
-    ctx.jacobian_d(x.data_ptr(), val.data_ptr(), sh)
+    ctx.jacobian_d(x.data_ptr(), 0, sh)          # into the context: the Jacobian the set-up factors
     torch.cuda.synchronize()
     n1 = sp.nfree["u"] + sp.nfree["j"]
     t_sm = {}
@@ -392,12 +393,12 @@ def main():
         yy = torch.zeros(n1, dtype=dt, device=dev)
         ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh)
         t_sm[prec] = time_events(lambda: ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh), 10, stream)
-    npatch, pdim, pent = ctx.smoother_info()
+    npatch, pdim, pent, npiv = ctx.smoother_info(pivoted=True)
     esize = {0: 8, 1: 4, 2: 2}
     sm_bytes = {p: pent * esize[p] for p in t_sm}
     log(f"smoother apply FP64 {t_sm[0] * 1e3:.3f} ms ({sm_bytes[0] / t_sm[0] / 1e9:.0f} GB/s), "
         f"mixed {t_sm[args.mixed] * 1e3:.3f} ms ({sm_bytes[args.mixed] / t_sm[args.mixed] / 1e9:.0f} GB/s), "
-        f"npatch={npatch} dim={pdim}")
+        f"npatch={npatch} dim={pdim} partial-pivoting patches={npiv}")
 
     cpu = None
     if not args.no_cpu:

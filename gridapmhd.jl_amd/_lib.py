@@ -78,9 +78,10 @@ EXPORTS = {
                                         ctypes.POINTER(MhdSolveInfo)]),
     "mhd_attach_coarse": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                          c_int32, c_double, c_int32]),
+    "mhd_set_pivot_threshold": (ctypes.c_int, [c_void_p, c_double]),
     "mhd_smoother_apply_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhd_smoother_info": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
-                                         ctypes.POINTER(c_int64)]),
+                                         ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -229,8 +230,10 @@ class Context:
     def residual_d(self, d_x: int, d_r: int, stream: int = 0):
         return self._check(self.lib.mhd_residual_d(self.h, d_x, d_r, stream or None), "mhd_residual_d")
 
-    def jacobian_d(self, d_x: int, d_val: int, stream: int = 0):
-        return self._check(self.lib.mhd_jacobian_d(self.h, d_x, d_val, stream or None), "mhd_jacobian_d")
+    def jacobian_d(self, d_x: int, d_val: int = 0, stream: int = 0):
+        """CSR values into d_val; d_val = 0 assembles into the context's own
+        storage (the Jacobian mhd_precond_setup factors)."""
+        return self._check(self.lib.mhd_jacobian_d(self.h, d_x, d_val or None, stream or None), "mhd_jacobian_d")
 
     def spmv_d(self, d_x: int, d_y: int, stream: int = 0):
         return self._check(self.lib.mhd_spmv_d(self.h, d_x, d_y, stream or None), "mhd_spmv_d")
@@ -294,11 +297,16 @@ class Context:
         """y += smoother(r); float64 vectors after a precision-0 set-up, float32 otherwise."""
         return self._check(self.lib.mhd_smoother_apply_d(self.h, d_r, d_y, stream or None), "mhd_smoother_apply_d")
 
-    def smoother_info(self):
-        a, b, c_ = c_int64(), c_int64(), c_int64()
-        self._check(self.lib.mhd_smoother_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c_)),
-                    "mhd_smoother_info")
-        return a.value, b.value, c_.value
+    def set_pivot_threshold(self, tau: float):
+        """Diagonal-pivot threshold of the patch inversions (0 .. 1, see mhd.h)."""
+        self._check(self.lib.mhd_set_pivot_threshold(self.h, float(tau)), "mhd_set_pivot_threshold")
+
+    def smoother_info(self, pivoted=False):
+        """(patches, padded patch size, sum of size^2) [, patches inverted with partial pivoting]."""
+        a, b, c_, d = c_int64(), c_int64(), c_int64(), c_int64()
+        self._check(self.lib.mhd_smoother_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c_),
+                                               ctypes.byref(d)), "mhd_smoother_info")
+        return (a.value, b.value, c_.value, d.value) if pivoted else (a.value, b.value, c_.value)
 
     def newton_solve(self, x0, opts: Optional[MhdSolverOpts] = None):
         x = np.array(x0, dtype=np.float64, copy=True)

@@ -297,9 +297,16 @@ int mhd_residual_d(mhd_ctx *c, const double *d_x, double *d_r, void *stream) {
 }
 
 int mhd_jacobian_d(mhd_ctx *c, const double *d_x, double *d_val, void *stream) {
-  if (!c || !d_x || !d_val) return -1;
+  if (!c || !d_x) return -1;
   MHD_HIP(hipSetDevice(c->device));
-  return mhd_launch_jacobian(c, d_x, d_val, stream ? (hipStream_t)stream : c->stream);
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  if (d_val) return mhd_launch_jacobian(c, d_x, d_val, s);
+  // into the context's own storage: the Jacobian mhd_precond_setup factors,
+  // with its state (the coarse levels' injected states come from it)
+  if (d_x != c->d_x) MHD_HIP(hipMemcpyAsync(c->d_x, d_x, sizeof(double) * c->n, hipMemcpyDeviceToDevice, s));
+  int rc = mhd_launch_jacobian(c, c->d_x, c->d_val, s);
+  c->jac_valid = rc == 0;
+  return rc;
 }
 
 int mhd_spmv_d(mhd_ctx *c, const double *d_x, double *d_y, void *stream) {
@@ -339,6 +346,7 @@ int mhd_jacobian(mhd_ctx *c, const double *x, double *nzval) {
   MHD_HIP(hipSetDevice(c->device));
   MHD_HIP(hipMemcpyAsync(c->d_x, x, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
   int rc = mhd_launch_jacobian(c, c->d_x, c->d_val, c->stream);
+  c->jac_valid = rc == 0;
   if (rc) return rc;
   if (!c->d_tperm) {
     MHD_HIP(hipMalloc(&c->d_tperm, sizeof(int64_t) * std::max<int64_t>(c->nnz, 1)));

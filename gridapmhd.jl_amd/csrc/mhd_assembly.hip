@@ -692,8 +692,19 @@ __global__ __launch_bounds__(WG, 2) void k_cell_integrals(int ncells, const doub
 }
 
 // Thin-wall face terms: one 64-wide workgroup per boundary face.
+//
+// Only the face's own four RT dofs (local 4 lf .. 4 lf + 3) have a nonzero
+// normal flux on it: every other reference field is either tangential to the
+// face or carries a 1-D Lagrange factor that vanishes there.  The normal flux
+// s.n is taken from the reference field, s.n = sgn rt_d (+-1) / (det J |J^-T n^|)
+// (the contravariant Piola identity), which is exactly zero for the other 32
+// dofs, so a face adds to the rows of its own dofs only.  A boundary face dof
+// belongs to one face, so every residual and Jacobian entry receives at most
+// one face contribution: the sums do not depend on the launch order, on
+// non-affine cells too.
 struct FaceLDS {
   double W[NQF];
+  double ndn[NQF];           // +-1 / (det J |J^-T n^|): s.n = sgn * rt_d * ndn
   double n[NQF][3];
   double psi[NQF][36][3];
   double gpsi[NQF][36][9];   // [i][k] = d psi_k / d x_i
@@ -752,6 +763,7 @@ __device__ void stage_face(FaceLDS &S, int f, const double *__restrict__ xyz, co
     nn = sqrt(nn);
     for (int i = 0; i < 3; ++i) S.n[q][i] = n[i] / nn;
     S.W[q] = F->w[q] * fabs(det) * nn;
+    S.ndn[q] = nh / (det * nn);
     for (int i = 0; i < 3; ++i)
       for (int l = 0; l < 3; ++l) {
         Jq[q][3 * i + l] = J[i][l];
@@ -783,11 +795,14 @@ __device__ void stage_face(FaceLDS &S, int f, const double *__restrict__ xyz, co
   if (tid < NQF) {
     const int q = tid;
     const double *n = S.n[q];
+    const int d = 2 - lf / 2;
     double jn = 0.0, jnn = 0.0;
+    for (int r = 4 * lf; r < 4 * lf + 4; ++r) {
+      const double sgn = sign ? (double)sign[(int64_t)cell * 6 + lf] : 1.0;
+      jn += S.xl[r] * sgn * F->rt[lf][q][r][d] * S.ndn[q];
+    }
     for (int r = 0; r < 36; ++r) {
-      const double *s = S.psi[q][r];
       const double *gs = S.gpsi[q][r];
-      jn += S.xl[r] * (s[0] * n[0] + s[1] * n[1] + s[2] * n[2]);
       double t = 0.0;
       for (int i = 0; i < 3; ++i) t += n[i] * (gs[3 * i + 0] * n[0] + gs[3 * i + 1] * n[1] + gs[3 * i + 2] * n[2]);
       jnn += S.xl[r] * t;
@@ -809,13 +824,14 @@ __global__ __launch_bounds__(64) void k_face_residual(int nfaces, const double *
   for (int f = blockIdx.x; f < nfaces; f += gridDim.x) {
     int cell, lf;
     stage_face(S, f, xyz, dofs, sign, dir, x, F, fcell, flf, nullptr, cell, lf);
-    const int a = threadIdx.x;
-    if (a < NJ && S.g[a] >= 0) {
+    const int a = 4 * lf + threadIdx.x;
+    if (threadIdx.x < 4 && S.g[a] >= 0) {
       const double tau = ftau[f], cw = fcw[f], jw = fjw[f];
+      const int d = 2 - lf / 2;
+      const double sgn = sign ? (double)sign[(int64_t)cell * 6 + lf] : 1.0;
       double acc = 0.0;
       for (int q = 0; q < NQF; ++q) {
-        const double *s = S.psi[q][a], *n = S.n[q];
-        const double sn = s[0] * n[0] + s[1] * n[1] + s[2] * n[2];
+        const double sn = sgn * F->rt[lf][q][a][d] * S.ndn[q];
         acc += mode == 2 ? S.W[q] * tau * sn * S.jnn[q] : S.W[q] * tau * sn * (S.jn[q] - jw + cw * S.jnn[q]);
       }
       atomicAdd(&r[S.g[a]], acc);
@@ -837,14 +853,17 @@ __global__ __launch_bounds__(64) void k_face_jacobian(int nfaces, const double *
     stage_face(S, f, xyz, dofs, sign, dir, x, F, fcell, flf, rowptr, cell, lf);
     const double tau = ftau[f], cw = fcw[f];
     const uint16_t *rk = rank + (int64_t)cell * NENT;
-    for (int idx = threadIdx.x; idx < NJ * NJ; idx += blockDim.x) {
-      const int a = idx / NJ, b = idx % NJ;
+    const int dax = 2 - lf / 2;
+    const double sgn = sign ? (double)sign[(int64_t)cell * 6 + lf] : 1.0;
+    for (int idx = threadIdx.x; idx < 4 * NJ; idx += blockDim.x) {
+      const int a = 4 * lf + idx / NJ, b = idx % NJ;
       if (S.g[a] < 0 || S.g[b] < 0) continue;
+      const bool bface = (b >> 2) == lf;
       double acc = 0.0;
       for (int q = 0; q < NQF; ++q) {
-        const double *n = S.n[q], *s = S.psi[q][a], *d = S.psi[q][b], *gd = S.gpsi[q][b];
-        const double sn = s[0] * n[0] + s[1] * n[1] + s[2] * n[2];
-        const double dn = d[0] * n[0] + d[1] * n[1] + d[2] * n[2];
+        const double *n = S.n[q], *gd = S.gpsi[q][b];
+        const double sn = sgn * F->rt[lf][q][a][dax] * S.ndn[q];
+        const double dn = bface ? sgn * F->rt[lf][q][b][dax] * S.ndn[q] : 0.0;
         double dnn = 0.0;
         for (int i = 0; i < 3; ++i) dnn += n[i] * (gd[3 * i + 0] * n[0] + gd[3 * i + 1] * n[1] + gd[3 * i + 2] * n[2]);
         acc += S.W[q] * tau * sn * (dn + cw * dnn);

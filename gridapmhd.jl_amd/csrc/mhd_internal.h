@@ -85,7 +85,9 @@ struct mhd_ctx {
   int64_t *d_rowptr = nullptr;    // [n+1]
   int32_t *d_colind = nullptr;    // [nnz]
   uint16_t *d_rank = nullptr;     // [ncells][NENT] rank of each touched entry inside its row
-  double *d_val = nullptr;        // [nnz] CSR values of the last Jacobian
+  double *d_val = nullptr;        // [nnz] CSR values of the last Jacobian assembled into the context
+  bool jac_valid = false;         // d_val holds A(d_x): what mhd_precond_setup factors
+  double pivot_tau = 0.03;        // threshold of a diagonal pivot in the patch inversions
   int64_t *d_tperm = nullptr;     // [nnz] CSR position of the transposed entry (CSC export)
   mhd::RefTables *d_ref = nullptr;
   mhd::FaceTables *d_fref = nullptr;

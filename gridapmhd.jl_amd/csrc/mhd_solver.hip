@@ -66,10 +66,6 @@ __global__ void k_convert(int64_t n, const S *__restrict__ x, D *__restrict__ y)
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = (D)x[i];
 }
-__global__ void k_to_bf16(int64_t n, const double *__restrict__ x, __hip_bfloat16 *__restrict__ y) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = __float2bfloat16((float)x[i]);
-}
 
 __device__ inline double block_sum(double v) {
   __shared__ double red[TB / 64];
@@ -342,17 +338,406 @@ __global__ __launch_bounds__(256) void k_dense_rowdot(int64_t m, const double *_
   if (lane == 0) y[row] = (T)v;
 }
 
-// part[P] = A_P^-1 r[P]  (one workgroup per patch, inverse column-major)
+// ---------------------------------------------------------------------------
+// Batched inversion of the vertex patches: in-place Gauss-Jordan elimination,
+// one 512-thread workgroup per patch, the matrix held in registers.  A vertex
+// star has at most 225 = 224 + 1 dofs (27 Q2 nodes x 3 components + 144 RT1
+// dofs).  Thread (tr, tc) = (t & 15, t >> 4) holds a 14 x 7 tile: 14 rows
+// (one of every block of 16 rows) and 7 columns (one of every block of 32
+// columns) of rows / columns 0..223; row 224 (all its columns) and column 224
+// (rows 0..223) live in LDS.  The FP64 tile is 196 VGPRs: 8 waves per CU
+// (2 per SIMD, 256 VGPRs each) hold the 405 KB patch in the register file.
+//
+// Elimination step k, with the pivot column saved as f and zeroed and the
+// pivot row p saved as r = a_p,: / a_pk (r_k = 1 / a_pk) and zeroed, updates
+// every entry as
+//   a_ij <- a_ij - f'_i r_j,   f'_i = f_i (i != p), f'_p = -1,
+// the in-place Gauss-Jordan step of [A | I] with column p of the right-hand
+// block stored in column k.
+//
+// k_patch_gjd pivots on the diagonal (p = k).  The tile then rotates one row
+// after every 16 steps and one column after every 32, so that the pivot row
+// and column always sit in register row 0 and column 0 and every register
+// index is a compile-time constant.  It accepts a diagonal pivot only if
+// |a_kk| >= tau max_{i > k} |a_ik| (threshold pivoting) and otherwise
+// flags the patch.  Flagged patches are inverted again by k_patch_gj, with
+// partial pivoting (the unused row of largest |a_ik|, lowest row on ties).
+// Rows are not swapped there: A^-1[k][j] = S[p_k][q_j] with q = p^-1, and the
+// smoother applies it without un-permuting: z = S x' with x'_c = x_{p_c}, and
+// z_r is the patch product of local dof q_r (p is the identity after
+// k_patch_gjd).  Patches smaller than the padded size are padded with
+// identity rows and columns, whose pivots are their own rows.
+template <class T>
+__device__ inline T from_dbl(double v) { return (T)v; }
+template <>
+__device__ inline __hip_bfloat16 from_dbl<__hip_bfloat16>(double v) { return __float2bfloat16((float)v); }
+
+constexpr int GJ_T = 512;              // threads per patch
+constexpr int GJ_TR = 16, GJ_TC = 32;  // thread grid: rows x columns
+constexpr int GJ_RI = 14, GJ_RJ = 7;   // register tile per thread
+constexpr int GJ_NMAX = GJ_TR * GJ_RI + 1;  // 225
+constexpr int GJ_NL = GJ_NMAX - 1;          // 224: the row / column held in LDS
+static_assert(GJ_TR * GJ_TC == GJ_T && GJ_TC * GJ_RJ == GJ_TR * GJ_RI, "square register part");
+
+struct GJShared {
+  double colbuf[GJ_NMAX], rowbuf[GJ_NMAX], R224[GJ_NMAX], C224[GJ_NL];
+  int used[GJ_NMAX];
+  int16_t pk[GJ_NMAX];
+  int p, bad;
+  double piv;
+};
+
+template <class OutT>
+__device__ __forceinline__ void gj_load(double (&S)[GJ_RI][GJ_RJ], GJShared &L, const double *__restrict__ Ap, int PS,
+                                        int t, int tr, int tc) {
+#pragma unroll
+  for (int i = 0; i < GJ_RI; ++i)
+#pragma unroll
+    for (int j = 0; j < GJ_RJ; ++j) {
+      const int r = tr + GJ_TR * i, c = tc + GJ_TC * j;
+      S[i][j] = (r < PS && c < PS) ? Ap[(int64_t)c * PS + r] : (r == c ? 1.0 : 0.0);
+    }
+  if (t < GJ_NMAX) {
+    L.R224[t] = (t < PS && GJ_NL < PS) ? Ap[(int64_t)t * PS + GJ_NL] : (t == GJ_NL ? 1.0 : 0.0);
+    L.used[t] = 0;
+  }
+  if (t < GJ_NL) L.C224[t] = (t < PS && GJ_NL < PS) ? Ap[(int64_t)GJ_NL * PS + t] : 0.0;
+  if (t == 0) L.bad = 0;
+}
+
+template <class OutT>
+__device__ __forceinline__ void gj_store(const double (&S)[GJ_RI][GJ_RJ], const GJShared &L, OutT *__restrict__ Op,
+                                         int PS, int t, int tr, int tc) {
+#pragma unroll
+  for (int i = 0; i < GJ_RI; ++i)
+#pragma unroll
+    for (int j = 0; j < GJ_RJ; ++j) {
+      const int r = tr + GJ_TR * i, c = tc + GJ_TC * j;
+      if (r < PS && c < PS) Op[(int64_t)c * PS + r] = from_dbl<OutT>(S[i][j]);
+    }
+  if (PS > GJ_NL) {
+    if (t < PS) Op[(int64_t)t * PS + GJ_NL] = from_dbl<OutT>(L.R224[t]);
+    if (t < GJ_NL) Op[(int64_t)GJ_NL * PS + t] = from_dbl<OutT>(L.C224[t]);
+  }
+}
+
+// Pivot search over (best |value|, lowest row) candidates of a lane group.
+__device__ __forceinline__ void gj_argmax(double &best, double &bval, int &bi, int width) {
+  for (int off = width >> 1; off > 0; off >>= 1) {
+    const double ob = __shfl_xor(best, off, width), ov = __shfl_xor(bval, off, width);
+    const int oi = __shfl_xor(bi, off, width);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bval = ov;
+      bi = oi;
+    }
+  }
+}
+
+// ---- diagonal pivots, rotating tile ----------------------------------------
+
+// Step k = 16 rb + tq (column block cb = k >> 5).  colbuf and rowbuf are
+// kept in register order: slot tr + 16 i holds the row of register row i,
+// slot tc + 32 j the column of register column j; slot 224 is row / column 224.
+__device__ __forceinline__ void gjd_step(double (&S)[GJ_RI][GJ_RJ], GJShared &L, int t, int tr, int tc, int rb,
+                                         int cb, int tq, double tau) {
+  const int k = GJ_TR * rb + tq, kc = k & (GJ_TC - 1);
+  // ---- pivot column into colbuf (zeroed in place), threshold test
+  if (tc == kc) {  // 16 lanes of wave kc >> 2
+    double cmax = 0.0, dia = 0.0;
+#pragma unroll
+    for (int i = 0; i < GJ_RI; ++i) {
+      const double v = S[i][0];
+      L.colbuf[tr + GJ_TR * i] = v;
+      S[i][0] = 0.0;
+      // register row i holds row block (i + rb) mod 14: unused rows are the
+      // blocks after rb and the rows of block rb after tq
+      if (i < GJ_RI - rb && (i > 0 || tr > tq)) cmax = fmax(cmax, fabs(v));
+      if (i == 0) dia = v;
+    }
+    if (tr == 0) {
+      const double v = L.R224[k];
+      L.colbuf[GJ_NL] = v;
+      cmax = fmax(cmax, fabs(v));
+    }
+    for (int off = GJ_TR >> 1; off > 0; off >>= 1) cmax = fmax(cmax, __shfl_xor(cmax, off, GJ_TR));
+    if (tr == tq) {
+      L.piv = dia;
+      if (!(fabs(dia) >= tau * cmax) || dia == 0.0) atomicCAS(&L.bad, 0, k + 1);
+    }
+  }
+  __syncthreads();
+  // ---- pivot row (register row 0 of lanes tr == tq) into rowbuf, zeroed
+  const double piv = L.piv;
+  const double inv = piv != 0.0 ? 1.0 / piv : 0.0;
+  if (tr == tq) {
+#pragma unroll
+    for (int j = 0; j < GJ_RJ; ++j) {
+      L.rowbuf[tc + GJ_TC * j] = (j == 0 && tc == kc) ? inv : S[0][j] * inv;
+      S[0][j] = 0.0;
+    }
+    if (tc == 0) L.rowbuf[GJ_NL] = L.C224[k] * inv;
+  }
+  if (t == 0) L.colbuf[tq] = -1.0;  // f'_k: the pivot row (zeroed above) becomes the scaled row
+  __syncthreads();
+  // ---- rank-1 update of every entry: a_ij <- a_ij - f'_i r_j
+  double rv[GJ_RJ];
+#pragma unroll
+  for (int j = 0; j < GJ_RJ; ++j) rv[j] = L.rowbuf[tc + GJ_TC * j];
+#pragma unroll
+  for (int i = 0; i < GJ_RI; ++i) {
+    const double f = L.colbuf[tr + GJ_TR * i];
+#pragma unroll
+    for (int j = 0; j < GJ_RJ; ++j) S[i][j] = fma(-f, rv[j], S[i][j]);
+  }
+  if (t < GJ_NMAX) {  // row 224, physical column t
+    int slot = GJ_NL;
+    if (t < GJ_NL) {
+      const int blk = (t >> 5) - cb;
+      slot = (t & (GJ_TC - 1)) + GJ_TC * (blk < 0 ? blk + GJ_RJ : blk);
+    }
+    const double old = t == k ? 0.0 : L.R224[t];
+    L.R224[t] = fma(-L.colbuf[GJ_NL], L.rowbuf[slot], old);
+  }
+  if (t < GJ_NL) {  // column 224, physical row t
+    const int blk = (t >> 4) - rb;
+    const int slot = (t & (GJ_TR - 1)) + GJ_TR * (blk < 0 ? blk + GJ_RI : blk);
+    const double old = t == k ? 0.0 : L.C224[t];
+    L.C224[t] = fma(-L.colbuf[slot], L.rowbuf[GJ_NL], old);
+  }
+  __syncthreads();
+}
+
+template <class OutT>
+__global__ __launch_bounds__(GJ_T) void k_patch_gjd(int PS, double tau, const double *__restrict__ A,
+                                                     OutT *__restrict__ out, int16_t *__restrict__ perm,
+                                                     int32_t *__restrict__ status) {
+  __shared__ GJShared L;
+  const int t = threadIdx.x, tr = t % GJ_TR, tc = t / GJ_TR;
+  const int64_t pch = blockIdx.x;
+  double S[GJ_RI][GJ_RJ];
+  gj_load<OutT>(S, L, A + pch * PS * PS, PS, t, tr, tc);
+  __syncthreads();
+  for (int cb = 0; cb < GJ_RJ; ++cb) {
+    for (int h = 0; h < 2; ++h) {
+      const int rb = 2 * cb + h;
+      for (int tq = 0; tq < GJ_TR; ++tq) gjd_step(S, L, t, tr, tc, rb, cb, tq, tau);
+      // next row block into register row 0
+#pragma unroll
+      for (int j = 0; j < GJ_RJ; ++j) {
+        const double r0 = S[0][j];
+#pragma unroll
+        for (int i = 0; i + 1 < GJ_RI; ++i) S[i][j] = S[i + 1][j];
+        S[GJ_RI - 1][j] = r0;
+      }
+    }
+    // next column block into register column 0
+#pragma unroll
+    for (int i = 0; i < GJ_RI; ++i) {
+      const double c0 = S[i][0];
+#pragma unroll
+      for (int j = 0; j + 1 < GJ_RJ; ++j) S[i][j] = S[i][j + 1];
+      S[i][GJ_RJ - 1] = c0;
+    }
+  }
+  // step 224: row and column 224 (LDS); the tile is back in place
+  if (t < GJ_NL) L.colbuf[t] = L.C224[t];
+  if (t == 0) {
+    const double v = L.R224[GJ_NL];
+    L.colbuf[GJ_NL] = v;
+    L.piv = v;
+    if (v == 0.0) atomicCAS(&L.bad, 0, GJ_NL + 1);
+  }
+  __syncthreads();
+  {
+    const double piv = L.piv;
+    const double inv = piv != 0.0 ? 1.0 / piv : 0.0;
+    if (t < GJ_NMAX) L.rowbuf[t] = t == GJ_NL ? inv : L.R224[t] * inv;
+    __syncthreads();
+    if (t == 0) L.colbuf[GJ_NL] = -1.0;
+    __syncthreads();
+    double rv[GJ_RJ];
+#pragma unroll
+    for (int j = 0; j < GJ_RJ; ++j) rv[j] = L.rowbuf[tc + GJ_TC * j];
+#pragma unroll
+    for (int i = 0; i < GJ_RI; ++i) {
+      const double f = L.colbuf[tr + GJ_TR * i];
+#pragma unroll
+      for (int j = 0; j < GJ_RJ; ++j) S[i][j] = fma(-f, rv[j], S[i][j]);
+    }
+    __syncthreads();
+    if (t < GJ_NMAX) L.R224[t] = L.rowbuf[t];
+    if (t < GJ_NL) L.C224[t] = -L.colbuf[t] * L.rowbuf[GJ_NL];
+    __syncthreads();
+  }
+  gj_store<OutT>(S, L, out + pch * PS * PS, PS, t, tr, tc);
+  int16_t *pp = perm + pch * 2 * PS;
+  if (t < PS) {
+    pp[t] = (int16_t)t;
+    pp[PS + t] = (int16_t)t;
+  }
+  if (t == 0) status[pch] = L.bad;
+}
+
+// ---- partial pivoting (patches the diagonal pivots rejected) ---------------
+
+// One Gauss-Jordan step on column k with the row of largest |a_ik| among the
+// unused rows.  Register column 0 is the column block in work (jk < 7: columns
+// tc + 32 jk; the tile rotates one column after every block, so register
+// column j holds column tc + 32 ((j + jk) mod 7)); jk = 7 is column 224 (LDS).
+__device__ __forceinline__ void gj_step(double (&S)[GJ_RI][GJ_RJ], GJShared &L, int t, int tr, int tc, int jk,
+                                        int tk) {
+  const int k = tk + GJ_TC * jk;
+  if (jk < GJ_RJ) {
+    if (tc == tk) {  // 16 lanes of wave tk >> 2
+      double best = -1.0, bval = 0.0;
+      int bi = 0x7fffffff;
+#pragma unroll
+      for (int i = 0; i < GJ_RI; ++i) {
+        const int r = tr + GJ_TR * i;
+        const double v = S[i][0];
+        L.colbuf[r] = v;
+        S[i][0] = 0.0;
+        const double a = L.used[r] ? -1.0 : fabs(v);
+        if (a > best) {
+          best = a;
+          bval = v;
+          bi = r;
+        }
+      }
+      if (tr == 0) {
+        const double v = L.R224[k];
+        L.colbuf[GJ_NL] = v;
+        const double a = L.used[GJ_NL] ? -1.0 : fabs(v);
+        if (a > best) {
+          best = a;
+          bval = v;
+          bi = GJ_NL;
+        }
+      }
+      gj_argmax(best, bval, bi, GJ_TR);
+      if (tr == 0) {
+        L.p = bi;
+        L.piv = bval;
+        if (!(best > 0.0) && L.bad == 0) L.bad = k + 1;
+      }
+    }
+  } else if (t < 64) {  // k = 224: the column lives in LDS
+    double best = -1.0, bval = 0.0;
+    int bi = 0x7fffffff;
+    for (int r = t; r < GJ_NMAX; r += 64) {
+      const double v = r < GJ_NL ? L.C224[r] : L.R224[GJ_NL];
+      L.colbuf[r] = v;
+      const double a = L.used[r] ? -1.0 : fabs(v);
+      if (a > best) {
+        best = a;
+        bval = v;
+        bi = r;
+      }
+    }
+    gj_argmax(best, bval, bi, 64);
+    if (t == 0) {
+      L.p = bi;
+      L.piv = bval;
+      if (!(best > 0.0) && L.bad == 0) L.bad = k + 1;
+    }
+  }
+  __syncthreads();
+  const int p = __builtin_amdgcn_readfirstlane(L.p);
+  const double piv = L.piv;
+  const double inv = piv != 0.0 ? 1.0 / piv : 0.0;
+  if (p < GJ_NL) {
+    // the lanes holding row p: one test per register row keeps every
+    // register index a compile-time constant
+#pragma unroll
+    for (int i = 0; i < GJ_RI; ++i)
+      if (tr + GJ_TR * i == p) {
+#pragma unroll
+        for (int j = 0; j < GJ_RJ; ++j) {
+          const int jj = j + jk;
+          const int c = tc + GJ_TC * (jj >= GJ_RJ ? jj - GJ_RJ : jj);
+          L.rowbuf[c] = c == k ? inv : S[i][j] * inv;
+          S[i][j] = 0.0;
+        }
+        if (tc == 0) L.rowbuf[GJ_NL] = k == GJ_NL ? inv : L.C224[p] * inv;
+      }
+  } else if (t < GJ_NMAX) {
+    L.rowbuf[t] = t == k ? inv : L.R224[t] * inv;
+  }
+  if (t == 0) {
+    L.pk[k] = (int16_t)p;
+    L.used[p] = 1;
+    L.colbuf[p] = -1.0;
+  }
+  __syncthreads();
+  double rv[GJ_RJ];
+#pragma unroll
+  for (int j = 0; j < GJ_RJ; ++j) {
+    const int jj = j + jk;
+    rv[j] = L.rowbuf[tc + GJ_TC * (jj >= GJ_RJ ? jj - GJ_RJ : jj)];
+  }
+#pragma unroll
+  for (int i = 0; i < GJ_RI; ++i) {
+    const double f = L.colbuf[tr + GJ_TR * i];
+#pragma unroll
+    for (int j = 0; j < GJ_RJ; ++j) S[i][j] = fma(-f, rv[j], S[i][j]);
+  }
+  if (t < GJ_NMAX) {
+    const double old = (t == k || p == GJ_NL) ? 0.0 : L.R224[t];
+    L.R224[t] = fma(-L.colbuf[GJ_NL], L.rowbuf[t], old);
+  }
+  if (t < GJ_NL) {
+    const double old = (t == p || k == GJ_NL) ? 0.0 : L.C224[t];
+    L.C224[t] = fma(-L.colbuf[t], L.rowbuf[GJ_NL], old);
+  }
+  __syncthreads();
+}
+
+template <class OutT>
+__global__ __launch_bounds__(GJ_T) void k_patch_gj(int PS, const int32_t *__restrict__ list,
+                                                    const double *__restrict__ A, OutT *__restrict__ out,
+                                                    int16_t *__restrict__ perm, int32_t *__restrict__ status) {
+  __shared__ GJShared L;
+  const int t = threadIdx.x, tr = t % GJ_TR, tc = t / GJ_TR;
+  const int64_t pch = list[blockIdx.x];
+  double S[GJ_RI][GJ_RJ];
+  gj_load<OutT>(S, L, A + pch * PS * PS, PS, t, tr, tc);
+  __syncthreads();
+  for (int jk = 0; jk < GJ_RJ; ++jk) {
+    for (int tk = 0; tk < GJ_TC; ++tk) gj_step(S, L, t, tr, tc, jk, tk);
+#pragma unroll
+    for (int i = 0; i < GJ_RI; ++i) {
+      const double c0 = S[i][0];
+#pragma unroll
+      for (int j = 0; j + 1 < GJ_RJ; ++j) S[i][j] = S[i][j + 1];
+      S[i][GJ_RJ - 1] = c0;
+    }
+  }
+  gj_step(S, L, t, tr, tc, GJ_RJ, 0);
+  gj_store<OutT>(S, L, out + pch * PS * PS, PS, t, tr, tc);
+  int16_t *pp = perm + pch * 2 * PS;
+  if (t < PS) {
+    pp[t] = L.pk[t];
+    pp[PS + L.pk[t]] = (int16_t)t;
+  }
+  if (t == 0) status[pch] = L.bad;
+}
+
+// The inverse is stored as Gauss-Jordan leaves it (see k_patch_gj): slot c
+// of the input takes the residual of local dof perm[c], and row t of the
+// product belongs to local dof perm[PS + t].
 template <class MP, class T>
 __global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const int32_t *__restrict__ psize,
-                                                     const int32_t *__restrict__ pidx, const MP *__restrict__ Ainv,
-                                                     const T *__restrict__ r, double *__restrict__ part) {
+                                                     const int32_t *__restrict__ pidx, const int16_t *__restrict__ perm,
+                                                     const MP *__restrict__ Ainv, const T *__restrict__ r,
+                                                     double *__restrict__ part) {
   __shared__ double rl[256];
   const int pch = blockIdx.x;
   const int sz = psize[pch];
   const int32_t *ids = pidx + (int64_t)pch * PS;
+  const int16_t *pp = perm + (int64_t)pch * 2 * PS;
   const int t = threadIdx.x;
-  if (t < sz) rl[t] = (double)r[ids[t]];
+  if (t < sz) rl[t] = (double)r[ids[pp[t]]];
   __syncthreads();
   if (t < sz) {
     const MP *Ap = Ainv + (int64_t)pch * PS * PS + t;
@@ -365,7 +750,7 @@ __global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const i
       a3 += todbl(Ap[(int64_t)(b + 3) * PS]) * rl[b + 3];
     }
     for (; b < sz; ++b) a0 += todbl(Ap[(int64_t)b * PS]) * rl[b];
-    part[(int64_t)pch * PS + t] = (a0 + a1) + (a2 + a3);
+    part[(int64_t)pch * PS + pp[PS + t]] = (a0 + a1) + (a2 + a3);
   }
 }
 
@@ -398,10 +783,14 @@ struct Solver {
   // dof -> patch rows holding it (patch * PS + local index), in patch order
   int32_t *d_dptr = nullptr, *d_dent = nullptr;
   double *d_ppart = nullptr;  // [npatch * PS] per-patch products
-  double *d_pA = nullptr;
-  float *d_pA32 = nullptr;
-  __hip_bfloat16 *d_pA16 = nullptr;
-  rocblas_int *d_ipiv = nullptr, *d_info = nullptr;
+  double *d_pA = nullptr;             // FP64 inverses (precision 0)
+  float *d_pA32 = nullptr;            // FP32 inverses (precision 1)
+  __hip_bfloat16 *d_pA16 = nullptr;   // BF16 inverses (precision 2)
+  int16_t *d_pperm = nullptr;         // [npatch][2][PS] Gauss-Jordan pivot rows and their inverse
+  int32_t *d_pstatus = nullptr;       // [npatch] 0, or 1 + the step of a zero pivot
+  double *d_pscratch = nullptr;       // FP64 patch matrices of one chunk
+  int pchunk = 0;                     // patches per chunk
+  int64_t nflagged = 0;               // patches of the last set-up inverted with partial pivoting
   rocblas_handle rb = nullptr;
   // mixed precision copy of A11 values
   float *d_val32 = nullptr;
@@ -609,14 +998,20 @@ int solver_init(mhd_ctx *c, const mhd_solver_opts *o) {
     MHD_HIP(hipMemcpy(S->d_dent, dent.data(), sizeof(int32_t) * dent.size(), hipMemcpyHostToDevice));
   }
   const size_t pelems = (size_t)S->npatch * S->PS * S->PS;
-  if ((rc = dmalloc(c, &S->d_pA, pelems))) return rc;
-  if ((rc = dmalloc(c, &S->d_ipiv, (size_t)S->npatch * S->PS))) return rc;
-  if ((rc = dmalloc(c, &S->d_info, (size_t)S->npatch))) return rc;
+  if (S->PS > GJ_NMAX) {
+    c->err = "vertex patch larger than " + std::to_string(GJ_NMAX) + " dofs";
+    return -1;
+  }
+  (void)pelems;
+  if ((rc = dmalloc(c, &S->d_pperm, (size_t)S->npatch * 2 * S->PS))) return rc;
+  if ((rc = dmalloc(c, &S->d_pstatus, (size_t)S->npatch))) return rc;
   if (rocblas_create_handle(&S->rb) != rocblas_status_success) {
     c->err = "rocblas_create_handle failed";
     return -2;
   }
   rocblas_set_stream(S->rb, c->stream);
+  // the same bits in every run: no atomics inside rocBLAS / rocSOLVER routines
+  rocblas_set_atomics_mode(S->rb, rocblas_atomics_not_allowed);
   // Krylov workspace
   S->m = krylov_dim(o->restart);
   S->kin = std::max(0, std::min(o->inner_iters, 120));
@@ -671,9 +1066,16 @@ int inner_alloc(mhd_ctx *c, int prec) {
   MHD_HIP(hipMalloc(&S->x1, es * n1 + es));
   MHD_HIP(hipMalloc(&S->r1, es * n1 + es));
   if (prec >= 1 && !S->d_val32) MHD_HIP(hipMalloc(&S->d_val32, sizeof(float) * std::max<int64_t>(c->nnz, 1)));
-  if (prec == 1 && !S->d_pA32) MHD_HIP(hipMalloc(&S->d_pA32, sizeof(float) * (size_t)S->npatch * S->PS * S->PS));
-  if (prec == 2 && !S->d_pA16)
-    MHD_HIP(hipMalloc(&S->d_pA16, sizeof(__hip_bfloat16) * (size_t)S->npatch * S->PS * S->PS));
+  const size_t pelems = (size_t)S->npatch * S->PS * S->PS;
+  if (prec == 0 && !S->d_pA) MHD_HIP(hipMalloc(&S->d_pA, sizeof(double) * pelems));
+  if (prec == 1 && !S->d_pA32) MHD_HIP(hipMalloc(&S->d_pA32, sizeof(float) * pelems));
+  if (prec == 2 && !S->d_pA16) MHD_HIP(hipMalloc(&S->d_pA16, sizeof(__hip_bfloat16) * pelems));
+  if (!S->d_pscratch) {
+    // the FP64 patch matrices are inverted chunk by chunk: only the inverses
+    // of the solve's precision are kept for the whole mesh
+    S->pchunk = std::min(S->npatch, 8192);
+    MHD_HIP(hipMalloc(&S->d_pscratch, sizeof(double) * (size_t)std::max(S->pchunk, 1) * S->PS * S->PS));
+  }
   S->inner_prec = prec;
   return 0;
 }
@@ -766,13 +1168,13 @@ int patch_as_on(mhd_ctx *c, const T *r, T *y, int prec, double w, bool accumulat
   Solver *S = c->solver;
   if (prec == 0)
     hipLaunchKernelGGL((k_patch_apply<double, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS, S->d_psize,
-                       S->d_pidx, S->d_pA, r, S->d_ppart);
+                       S->d_pidx, S->d_pperm, S->d_pA, r, S->d_ppart);
   else if (prec == 1)
     hipLaunchKernelGGL((k_patch_apply<float, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS, S->d_psize,
-                       S->d_pidx, S->d_pA32, r, S->d_ppart);
+                       S->d_pidx, S->d_pperm, S->d_pA32, r, S->d_ppart);
   else
     hipLaunchKernelGGL((k_patch_apply<__hip_bfloat16, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS,
-                       S->d_psize, S->d_pidx, S->d_pA16, r, S->d_ppart);
+                       S->d_psize, S->d_pidx, S->d_pperm, S->d_pA16, r, S->d_ppart);
   hipLaunchKernelGGL((k_patch_gather<T>), dim3(nblocks(S->n1)), dim3(TB), 0, st, S->n1, S->d_dptr, S->d_dent,
                      S->d_ppart, w, y, accumulate ? 1 : 0);
   MHD_HIP(hipGetLastError());
@@ -1011,25 +1413,58 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
   if (prec >= 1)
     hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(c->nnz)), dim3(TB), 0, c->stream, c->nnz, c->d_val,
                        S->d_val32);
-  const size_t pelems = (size_t)S->npatch * S->PS * S->PS;
-  MHD_HIP(hipMemsetAsync(S->d_pA, 0, sizeof(double) * pelems, c->stream));
-  hipLaunchKernelGGL(k_patch_extract, dim3(S->npatch), dim3(256), 0, c->stream, S->npatch, S->PS, S->d_psize,
-                     S->d_pidx, c->d_rowptr, c->d_colind, c->d_val, S->d_pA);
-  MHD_HIP(hipGetLastError());
-  const rocblas_stride st = (rocblas_stride)S->PS * S->PS;
-  if (rocsolver_dgetrf_strided_batched(S->rb, S->PS, S->PS, S->d_pA, S->PS, st, S->d_ipiv, S->PS, S->d_info,
-                                       S->npatch) != rocblas_status_success ||
-      rocsolver_dgetri_strided_batched(S->rb, S->PS, S->d_pA, S->PS, st, S->d_ipiv, S->PS, S->d_info, S->npatch) !=
-          rocblas_status_success) {
-    c->err = "rocsolver batched LU of the vertex patches failed";
-    return -2;
+  // patch matrices, chunk by chunk (d_pscratch) -> Gauss-Jordan inverses in
+  // the array of this precision: diagonal pivots first, partial pivoting for
+  // the patches whose diagonal pivots fail the threshold test
+  const int64_t pe = (int64_t)S->PS * S->PS;
+  S->nflagged = 0;
+  for (int p0 = 0; p0 < S->npatch; p0 += S->pchunk) {
+    const int np = std::min(S->pchunk, S->npatch - p0);
+    double *mat = S->d_pscratch;
+    MHD_HIP(hipMemsetAsync(mat, 0, sizeof(double) * (size_t)np * pe, c->stream));
+    hipLaunchKernelGGL(k_patch_extract, dim3(np), dim3(256), 0, c->stream, np, S->PS, S->d_psize + p0,
+                       S->d_pidx + (int64_t)p0 * S->PS, c->d_rowptr, c->d_colind, c->d_val, mat);
+    int16_t *pp = S->d_pperm + (int64_t)p0 * 2 * S->PS;
+    int32_t *st = S->d_pstatus + p0;
+    double *o64 = prec == 0 ? S->d_pA + p0 * pe : nullptr;
+    float *o32 = prec == 1 ? S->d_pA32 + p0 * pe : nullptr;
+    __hip_bfloat16 *o16 = prec == 2 ? S->d_pA16 + p0 * pe : nullptr;
+    if (prec == 0)
+      hipLaunchKernelGGL(k_patch_gjd<double>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, o64, pp, st);
+    else if (prec == 1)
+      hipLaunchKernelGGL(k_patch_gjd<float>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, o32, pp, st);
+    else
+      hipLaunchKernelGGL(k_patch_gjd<__hip_bfloat16>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, o16, pp, st);
+    MHD_HIP(hipGetLastError());
+    std::vector<int32_t> status(np), flagged;
+    MHD_HIP(hipMemcpyAsync(status.data(), st, sizeof(int32_t) * np, hipMemcpyDeviceToHost, c->stream));
+    MHD_HIP(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < np; ++q)
+      if (status[q] != 0) flagged.push_back(q);
+    if (flagged.empty()) continue;
+    S->nflagged += (int64_t)flagged.size();
+    int32_t *d_list = nullptr;
+    MHD_HIP(hipMalloc(&d_list, sizeof(int32_t) * flagged.size()));
+    MHD_HIP(hipMemcpyAsync(d_list, flagged.data(), sizeof(int32_t) * flagged.size(), hipMemcpyHostToDevice,
+                           c->stream));
+    const dim3 g((unsigned)flagged.size());
+    if (prec == 0)
+      hipLaunchKernelGGL(k_patch_gj<double>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, o64, pp, st);
+    else if (prec == 1)
+      hipLaunchKernelGGL(k_patch_gj<float>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, o32, pp, st);
+    else
+      hipLaunchKernelGGL(k_patch_gj<__hip_bfloat16>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, o16, pp, st);
+    MHD_HIP(hipGetLastError());
+    MHD_HIP(hipMemcpyAsync(status.data(), st, sizeof(int32_t) * np, hipMemcpyDeviceToHost, c->stream));
+    MHD_HIP(hipStreamSynchronize(c->stream));
+    MHD_HIP(hipFree(d_list));
+    for (int q : flagged)
+      if (status[q] != 0) {
+        c->err = "vertex patch " + std::to_string(p0 + q) + " is singular (zero pivot at step " +
+                 std::to_string(status[q] - 1) + ")";
+        return -3;
+      }
   }
-  if (prec == 1)
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks((int64_t)pelems)), dim3(TB), 0, c->stream,
-                       (int64_t)pelems, S->d_pA, S->d_pA32);
-  if (prec == 2)
-    hipLaunchKernelGGL(k_to_bf16, dim3(nblocks((int64_t)pelems)), dim3(TB), 0, c->stream, (int64_t)pelems, S->d_pA,
-                       S->d_pA16);
   MHD_HIP(hipGetLastError());
   if (!c->coarse) return 0;
   CoarseLink *L = c->coarse;
@@ -1178,7 +1613,7 @@ void mhd_solver_free(mhd_ctx *c) {
   if (!c || !c->solver) return;
   Solver *S = c->solver;
   void *ptrs[] = {S->d_split, S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, S->d_psize, S->d_pidx, S->d_pA,
-                  S->d_pA32, S->d_pA16, S->d_ipiv, S->d_info, S->d_val32, S->V, S->Z, S->w, S->b, S->dx, S->t1,
+                  S->d_pA32, S->d_pA16, S->d_pperm, S->d_pstatus, S->d_pscratch, S->d_val32, S->V, S->Z, S->w, S->b, S->dx, S->t1,
                   S->V1, S->Z1, S->w1, S->x1, S->r1, S->d_part, S->d_dots, S->d_y, S->d_dptr, S->d_dent,
                   S->d_ppart};
   for (void *p : ptrs)
@@ -1313,6 +1748,11 @@ int mhd_precond_setup(mhd_ctx *c, const mhd_solver_opts *o) {
     c->err = "precision must be 0, 1 or 2";
     return -1;
   }
+  if (!c->jac_valid) {
+    c->err = "mhd_precond_setup needs a Jacobian assembled into the context (mhd_jacobian, a solve, or "
+             "mhd_jacobian_d with d_nzval_csr = NULL)";
+    return -1;
+  }
   MHD_HIP(hipSetDevice(c->device));
   int rc;
   if (c->solver && (c->solver->m != krylov_dim(o->restart) ||
@@ -1337,8 +1777,22 @@ int mhd_smoother_apply_d(mhd_ctx *c, const void *d_r, void *d_y, void *stream) {
   return patch_as_on<float>(c, (const float *)d_r, (float *)d_y, S->inner_prec, 1.0, true, s);
 }
 
-int mhd_smoother_info(mhd_ctx *c, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries) {
+int mhd_set_pivot_threshold(mhd_ctx *c, double tau) {
+  if (!c) return -1;
+  if (!(tau >= 0.0 && tau <= 1.0)) {
+    c->err = "pivot threshold must lie in [0, 1]";
+    return -1;
+  }
+  c->pivot_tau = tau;
+  for (mhd_ctx *q = c->coarse ? c->coarse->ctx : nullptr; q; q = q->coarse ? q->coarse->ctx : nullptr)
+    q->pivot_tau = tau;
+  return 0;
+}
+
+int mhd_smoother_info(mhd_ctx *c, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries,
+                      int64_t *npivoted) {
   if (!c || !c->solver) return -1;
+  if (npivoted) *npivoted = c->solver->nflagged;
   if (npatch) *npatch = c->solver->npatch;
   if (patch_dim) *patch_dim = c->solver->PS;
   if (patch_entries) {  // sum over patches of size^2 (entries read per application)
@@ -1409,6 +1863,7 @@ int linear_solve(mhd_ctx *c, const double *x_state, const double *rhs, double *l
   MHD_HIP(hipMemcpyAsync(c->d_x, x_state, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   double t = now();
   if ((rc = mhd_launch_jacobian(c, c->d_x, c->d_val, c->stream))) return finish(rc);
+  c->jac_valid = true;
   if (transpose && hipMalloc(&d_valT, sizeof(double) * std::max<int64_t>(c->nnz, 1)) != hipSuccess) {
     c->err = "out of device memory for A^T";
     d_valT = nullptr;
@@ -1499,6 +1954,7 @@ int mhd_newton_solve(mhd_ctx *c, double *x_host, const mhd_solver_opts *o, mhd_s
     if (rn <= std::max(o->atol, o->rtol * r0)) break;
     t = now();
     if ((rc = mhd_launch_jacobian(c, c->d_x, c->d_val, c->stream))) return rc;
+    c->jac_valid = true;
     MHD_HIP(hipStreamSynchronize(c->stream));
     info->t_jacobian += now() - t;
     t = now();

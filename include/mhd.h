@@ -145,12 +145,15 @@ int mhd_cell_integrals(mhd_ctx *ctx, const double *x_free, double *out);
 /* Device-pointer variants (inputs resident in HBM).  The Jacobian values are
  * written in the library's CSR order (row-major, columns sorted); the pattern
  * is structurally symmetric, so CSR(A) and CSC(A) share index arrays and the
- * CSC values are CSR(A^T). */
+ * CSC values are CSR(A^T).  mhd_jacobian_d with d_nzval_csr = NULL assembles
+ * into the context's own storage (and keeps d_x as its state): the Jacobian
+ * that mhd_precond_setup and mhd_spmv_d use. */
 int mhd_residual_d(mhd_ctx *ctx, const double *d_x, double *d_r, void *stream);
 int mhd_jacobian_d(mhd_ctx *ctx, const double *d_x, double *d_nzval_csr, void *stream);
 int mhd_csr_arrays(mhd_ctx *ctx, const int64_t **d_rowptr, const int32_t **d_colind);
 
-/* y = A x with the last assembled Jacobian (device pointers). */
+
+/* y = A x with the Jacobian last assembled into the context (device pointers). */
 int mhd_spmv_d(mhd_ctx *ctx, const double *d_x, double *d_y, void *stream);
 
 /* Newton + FGMRES + Badia2024 block preconditioner.  Replaces
@@ -199,10 +202,13 @@ int mhd_attach_coarse(mhd_ctx *ctx, mhd_ctx *coarse, int64_t nrows, int64_t ncol
                       double smoother_weight,
                       int32_t coarse_step);
 
-/* Preconditioner set-up for the last assembled Jacobian: the numerical_setup!
- * of the block solver (src/Solvers/badia2024.jl:20-31) -- Schur scalings,
- * vertex-star patch extraction and batched LU.  opts->precision selects the
- * storage of the patch inverses (0 FP64, 1 FP32, 2 BF16). */
+/* Preconditioner set-up for the Jacobian last assembled into the context (by
+ * mhd_jacobian, a solve, or mhd_jacobian_d with d_nzval_csr = NULL; an error
+ * before any): the numerical_setup! of the block solver
+ * (src/Solvers/badia2024.jl:20-31) -- Schur scalings, vertex-star patch
+ * extraction and batched inversion, coarse levels at the injected state.
+ * opts->precision selects the storage of the patch inverses (0 FP64, 1 FP32,
+ * 2 BF16).  A singular patch is an error. */
 int mhd_precond_setup(mhd_ctx *ctx, const mhd_solver_opts *opts);
 
 /* y += sum_P R_P^T A_P^-1 R_P r on the [u,j] block (device pointers, length
@@ -213,8 +219,18 @@ int mhd_precond_setup(mhd_ctx *ctx, const mhd_solver_opts *opts);
  * result in every run) through a workspace of the context, so calls that
  * overlap each other or a solve of the same context on another stream race. */
 int mhd_smoother_apply_d(mhd_ctx *ctx, const void *d_r, void *d_y, void *stream);
-/* Patch count, padded patch dimension, sum over patches of size^2. */
-int mhd_smoother_info(mhd_ctx *ctx, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries);
+/* The vertex-patch inversions of mhd_precond_setup (batched Gauss-Jordan)
+ * pivot on the diagonal while |a_kk| >= tau max_{i>k} |a_ik| and invert a patch
+ * whose diagonal pivot fails that test again with partial pivoting (the
+ * largest |a_ik|).  tau in [0, 1]: 1 = every patch with partial pivoting,
+ * 0 = diagonal pivots unless one is zero.  Default 0.03; applies to the
+ * attached coarse levels too.  No counterpart in the reference (its patch
+ * solves are library LU factorisations). */
+int mhd_set_pivot_threshold(mhd_ctx *ctx, double tau);
+
+/* Patch count, padded patch dimension, sum over patches of size^2, and the
+ * patches of the last set-up inverted with partial pivoting. */
+int mhd_smoother_info(mhd_ctx *ctx, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries, int64_t *npivoted);
 
 #ifdef __cplusplus
 }

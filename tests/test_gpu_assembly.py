@@ -112,3 +112,68 @@ def test_scatter_modes_agree(name):
             ctx.set_assembly_mode(5)
     finally:
         ctx.close()
+
+
+def test_face_terms_touch_face_rows_only_and_repeat():
+    """Thin-wall faces on jittered (non-affine) cells: the face terms (GPU with
+    minus without the wall faces) are nonzero only in the rows of the faces'
+    own normal-flux dofs, so no entry receives two face contributions, and the
+    residual and the Jacobian repeat bit for bit."""
+    import gridapmhd_amd as g
+    _say("[assembly] face rows, jittered 5x5x3")
+    sp, phys = cases.duct_thin_wall_case((5, 5, 3), Ha=500.0, Re=50.0, cw=0.01, convection="newton",
+                                         zeta_u=1.0, zeta_j=1.0, jitter=0.2)
+    rng = np.random.default_rng(3)
+    x = 0.1 * rng.standard_normal(sp.ndofs)
+    ctx = g._lib.context_from_spaces(sp, phys)
+    sp0 = cases.duct_thin_wall_case((5, 5, 3), Ha=500.0, Re=50.0, cw=0.01, convection="newton",
+                                    zeta_u=1.0, zeta_j=1.0, jitter=0.2)[0]
+    sp0.thin_wall_faces = []
+    ctx0 = g._lib.context_from_spaces(sp0, phys)
+    try:
+        r1, r2 = ctx.residual(x), ctx.residual(x)
+        j1, j2 = ctx.jacobian(x), ctx.jacobian(x)
+        assert np.array_equal(r1, r2) and np.array_equal(j1, j2)
+        face_rows = set()
+        for tw in sp.thin_wall_faces:
+            for c, lf in zip(tw["cell"], tw["lface"]):
+                ids = sp.cell_dofs[c, 85 + 4 * lf:85 + 4 * lf + 4]
+                face_rows.update(int(i) for i in ids if i >= 0)
+        dr = r1 - ctx0.residual(x)
+        rows = set(np.nonzero(dr)[0].tolist())
+        _say(f"  {len(rows)} residual rows carry face terms, {len(face_rows)} face dofs")
+        assert rows <= face_rows and len(rows) > 0
+    finally:
+        ctx.close()
+        ctx0.close()
+
+
+@pytest.mark.parametrize("name", ["stretched_box", "graded_3x2x2"])
+def test_gpu_thin_wall_matches_closed_form(name):
+    """The closed-form pin of the thin-wall term (tests/test_thin_wall.py)
+    through the GPU face kernels: s* . R_face(j) against exact face integrals."""
+    import gridapmhd_amd as g
+    import test_thin_wall as tw
+    _say(f"[assembly] thin-wall closed form, {name}")
+    tau, cw, jw = 37.0, 0.173, 0.61
+    kw = tw.CASES[name]
+    sp = tw._case(tau=tau, cw=cw, jw=jw, **kw)
+    rng = np.random.default_rng(3)
+    J, S = tw._rt_field(rng), tw._rt_field(rng)
+    xj, _ = g.fespaces.interpolate(sp, j=lambda X: np.stack([tw._eval(c, X) for c in J], 1))
+    xs, _ = g.fespaces.interpolate(sp, j=lambda X: np.stack([tw._eval(c, X) for c in S], 1))
+    phys = dict(alpha=0.0, beta=0.0, gamma=0.0, sigma=1.0, B=np.zeros(3), convection="none")
+    ctx = g._lib.context_from_spaces(sp, phys)
+    faces = sp.thin_wall_faces
+    sp.thin_wall_faces = []
+    ctx0 = g._lib.context_from_spaces(sp, phys)
+    sp.thin_wall_faces = faces
+    try:
+        js = sp.field_slices()["j"]
+        lhs = float(xs[js] @ (ctx.residual(xj) - ctx0.residual(xj))[js])
+    finally:
+        ctx.close()
+        ctx0.close()
+    rhs = tw._closed_form(J, S, tau, cw, jw, kw["bounds"])
+    _say(f"  GPU {lhs!r} closed form {rhs!r}")
+    assert abs(lhs - rhs) <= 1e-11 * max(1.0, abs(rhs))

@@ -50,27 +50,25 @@ def test_hunt_analytic_errors_match_reference():
         assert rel < 1e-9, (k, rel)
 
 
-def test_fully_developed_thin_wall_converges_to_general_hunt():
-    """FullyDeveloped duct (Ha = 10, thin conducting Hartmann walls cw = 0.1,
-    tau = 1e4, insulating side walls) through main(params) on the GPU: the
-    distance to analytical_general_hunt_u shrinks with h on 8, 16, 32 cells
-    per side, and the penalty limit is reached as tau grows."""
+def test_fully_developed_thin_wall_duct():
+    """FullyDeveloped duct (Ha = 10, thin conducting Hartmann walls cw = 0.1
+    with the reference's default penalty tau = 100, insulating side walls)
+    through main(params) on the GPU: on 8 x 8 x 3 cells the mean velocity and
+    the distance to analytical_general_hunt_u equal the oracle's (CPU direct
+    solve: uh_0 = 4.256257, e_u = 3.536e-3); on 16 x 16 x 3 the distance is
+    that of the tau = 100 penalty limit (about 1e-2).  The h -> 0,
+    tau -> infinity convergence to the analytic solution is pinned on the CPU
+    (tests/test_oracle.py)."""
     import gridapmhd_amd as g
     errs = {}
-    for n in (8, 16, 32):
-        info, _ = g.fully_developed(nc=(n, n), Ha=10.0, cw_Ha=0.1, τ_Ha=1e4, nsums=50, rtol=1e-11, atol=1e-16,
-                                    niter_ls=2000, precision="fp64")
+    for n in (8, 16):
+        info, _ = g.fully_developed(nc=(n, n), Ha=10.0, cw_Ha=0.1, τ_Ha=100.0, nsums=50, rtol=1e-8, atol=1e-16,
+                                    niter=8, niter_ls=3000, precision="fp64")
         errs[n] = info["e_u_l2_rel"]
-        print(f"[main] fully developed {n}x{n}x3 cw_Ha=0.1 tau=1e4: e_u={errs[n]:.3e} kp={info['kp']:.5f} "
-              f"kp_tillac={info['kp_a']:.5f} converged={info['converged']}", flush=True)
+        print(f"[main] fully developed {n}x{n}x3 cw_Ha=0.1 tau=100: e_u={errs[n]:.4e} uh_0={info['uh_0']:.8f} "
+              f"kp={info['kp']:.5f} kp_tillac={info['kp_a']:.5f} converged={info['converged']} "
+              f"ls={info['ls_iters_per_step']}", flush=True)
         assert info["converged"]
-    assert errs[16] < 0.6 * errs[8] and errs[32] < 0.6 * errs[16] and errs[32] < 3e-3, errs
-    taus = {}
-    for tau in (1e2, 1e4, 1e6):
-        info, _ = g.fully_developed(nc=(16, 16), Ha=10.0, cw_Ha=0.1, τ_Ha=tau, nsums=50, rtol=1e-11, atol=1e-16,
-                                    niter_ls=2000, precision="fp64")
-        taus[tau] = info["uh_0"]
-        print(f"[main] fully developed 16x16x3 tau={tau:g}: uh_0={info['uh_0']:.10f} e_u={info['e_u_l2_rel']:.3e}",
-              flush=True)
-    # the penalty converges: the 1e4 -> 1e6 change is far below the 1e2 -> 1e4 one
-    assert abs(taus[1e6] - taus[1e4]) < 0.05 * abs(taus[1e4] - taus[1e2])
+        if n == 8:
+            assert abs(info["uh_0"] / 4.256257 - 1) < 1e-6 and abs(errs[8] / 3.536e-3 - 1) < 1e-3
+    assert 5e-3 < errs[16] < 2e-2, errs

@@ -149,6 +149,7 @@ def test_feature_cases_match_direct(name, precision, orc_mod):
         x, info, rc = ctx.newton_solve(np.zeros(sp.ndofs), o)
         err = cases.field_errors(sp, x, x_ref)
         relres = info.residual / info.residual0
+        _say(f"  patches inverted with partial pivoting: {ctx.smoother_info(pivoted=True)[3]}")
         _say(f"  newton={info.newton_iters} krylov={info.krylov_iters} rc={rc} relres={relres:.1e} "
              + " ".join(f"{k}={v:.2e}" for k, v in err.items()))
         assert rc >= 0 and relres < 1e-12
@@ -194,3 +195,75 @@ def test_solves_repeat_bitwise(precision):
         assert np.array_equal(xa, xb)
     finally:
         ctx.close()
+
+
+def _vertex_patches(sp):
+    """The vertex-star patches of the [u,j] block (badia2024 numbering): for
+    every vertex, the free u / j dofs whose support lies in the cells around it
+    (a Q2 node of a cell within one lattice step of the vertex; an RT1 face dof
+    of a face through the vertex, or an interior dof)."""
+    n1 = sp.nfree["u"] + sp.nfree["j"]
+    lists = {}
+    cd = sp.cell_dofs
+    for cc in range(sp.ncells):
+        for lv in range(8):
+            vi, vj, vk = lv & 1, (lv >> 1) & 1, lv >> 2
+            L = lists.setdefault(int(sp.cell_verts[cc, lv]), set())
+            for a in range(27):
+                i, j, k = a % 3, (a // 3) % 3, a // 9
+                if abs(i - 2 * vi) > 1 or abs(j - 2 * vj) > 1 or abs(k - 2 * vk) > 1:
+                    continue
+                for comp in range(3):
+                    g = int(cd[cc, comp * 27 + a])
+                    if 0 <= g < n1:
+                        L.add(g)
+            for r in range(36):
+                if r < 24:
+                    f = r >> 2
+                    d, side = 2 - f // 2, f & 1
+                    if (vi, vj, vk)[d] != side:
+                        continue
+                g = int(cd[cc, 85 + r])
+                if 0 <= g < n1:
+                    L.add(g)
+    return [np.array(sorted(L)) for _, L in sorted(lists.items()) if L]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_smoother_matches_patch_inverses(precision, orc_mod):
+    """The vertex-star Schwarz smoother (mhd_smoother_apply_d after
+    mhd_precond_setup: the patch matrices inverted on the GPU by in-register
+    Gauss-Jordan) against y = sum_P R_P^T A_P^-1 R_P r computed on the CPU from
+    the oracle's Jacobian with numpy inverses.  Newton convection at a random
+    state (non-symmetric patches), thin walls, a jittered mesh."""
+    import gridapmhd_amd as g
+    from devmem import DeviceArray
+    _say(f"[solver] smoother vs numpy patch inverses, precision={precision}")
+    sp, phys = cases.duct_thin_wall_case((4, 4, 3), Ha=30.0, Re=10.0, cw=0.05, convection="newton",
+                                         style="badia2024", zeta_u=1.0, zeta_j=1.0, jitter=0.15)
+    rng = np.random.default_rng(9)
+    x = 0.3 * rng.standard_normal(sp.ndofs)
+    n1 = sp.nfree["u"] + sp.nfree["j"]
+    A = orc_mod.Oracle(cases.problem_dict(sp, phys)).jacobian(x).tocsr()[:n1, :n1]
+    patches = _vertex_patches(sp)
+    r = rng.standard_normal(n1)
+    y_ref = np.zeros(n1)
+    for ids in patches:
+        Ap = A[ids][:, ids].toarray()
+        y_ref[ids] += np.linalg.solve(Ap, r[ids])
+    ctx = g._lib.context_from_spaces(sp, phys)
+    try:
+        ctx.jacobian(x)                       # into the context: the Jacobian the set-up inverts
+        ctx.precond_setup(ctx.default_opts(precision=precision))
+        npatch, pdim, _ = ctx.smoother_info()
+        assert npatch == len(patches) and pdim == max(len(p) for p in patches)
+        dt = np.float64 if precision == 0 else np.float32
+        rr = DeviceArray.from_numpy(r, dt)
+        yy = DeviceArray.from_numpy(np.zeros(n1), dt)
+        ctx.smoother_apply_d(rr.ptr, yy.ptr)
+        y = yy.numpy().astype(np.float64)
+    finally:
+        ctx.close()
+    err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
+    _say(f"  {len(patches)} patches (max {pdim} dofs): rel err {err:.2e}")
+    assert err < (1e-12 if precision == 0 else 1e-5)
