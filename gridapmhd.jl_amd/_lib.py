@@ -64,6 +64,7 @@ EXPORTS = {
     "mhd_jacobian": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "mhd_cell_integrals": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
     "mhd_residual_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mhd_residual_d_noalloc": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhd_jacobian_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhd_csr_arrays": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]),
     "mhd_spmv_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -229,6 +230,11 @@ class Context:
     # ---- device-pointer entry points (pointers as ints) ---------------------
     def residual_d(self, d_x: int, d_r: int, stream: int = 0):
         return self._check(self.lib.mhd_residual_d(self.h, d_x, d_r, stream or None), "mhd_residual_d")
+
+    def residual_d_noalloc(self, d_x: int, d_r: int, stream: int = 0):
+        """mhd_residual_d through the library's fixed device workspace (no allocation)."""
+        return self._check(self.lib.mhd_residual_d_noalloc(self.h, d_x, d_r, stream or None),
+                           "mhd_residual_d_noalloc")
 
     def jacobian_d(self, d_x: int, d_val: int = 0, stream: int = 0):
         """CSR values into d_val; d_val = 0 assembles into the context's own

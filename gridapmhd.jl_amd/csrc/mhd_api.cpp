@@ -224,7 +224,9 @@ int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, co
   MHD_HIP(hipMalloc(&c->d_val, sizeof(double) * std::max<int64_t>(c->nnz, 1)));
   MHD_HIP(hipMalloc(&c->d_x, sizeof(double) * std::max<int64_t>(n, 1)));
   MHD_HIP(hipMalloc(&c->d_r, sizeof(double) * std::max<int64_t>(n, 1)));
-  return 0;
+  // device address of this GPU's allocation-free residual workspace (looked up
+  // once per device, with the context's device current)
+  return mhd_noalloc_init(c);
 }
 
 void mhd_destroy(mhd_ctx *c) {
@@ -294,6 +296,12 @@ int mhd_residual_d(mhd_ctx *c, const double *d_x, double *d_r, void *stream) {
   if (!c || !d_x || !d_r) return -1;
   MHD_HIP(hipSetDevice(c->device));
   return mhd_launch_residual(c, d_x, d_r, stream ? (hipStream_t)stream : c->stream);
+}
+
+int mhd_residual_d_noalloc(mhd_ctx *c, const double *d_x, double *d_r, void *stream) {
+  if (!c || !d_x || !d_r) return -1;
+  MHD_HIP(hipSetDevice(c->device));
+  return mhd_launch_residual_noalloc(c, d_x, d_r, stream ? (hipStream_t)stream : c->stream);
 }
 
 int mhd_jacobian_d(mhd_ctx *c, const double *d_x, double *d_val, void *stream) {

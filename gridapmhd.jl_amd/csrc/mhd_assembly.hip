@@ -13,6 +13,8 @@
 // entries of the element matrix are spread over the workgroup in the
 // canonical row-major order of the cell's rank map, so the rank-map reads and
 // the row-local scatter into the CSR values are contiguous.
+#include <mutex>
+
 #include "mhd_internal.h"
 
 using namespace mhd;
@@ -1096,6 +1098,73 @@ int mhd_launch_residual_mode(mhd_ctx *c, const double *d_x, double *d_r, hipStre
 
 int mhd_launch_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s) {
   return mhd_launch_residual_mode(c, d_x, d_r, s, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free residual.  Its element residuals live in `scratch`, device
+// memory of the code object: every GPU the library runs on holds its own
+// copy (NOALLOC_DOUBLES doubles = 128 MiB) until the process ends.  Host code
+// never passes `scratch` itself to a kernel (on the host it only names the
+// symbol): the device address of each GPU's copy is looked up once per device
+// with hipGetSymbolAddress while that device is current, and that pointer is
+// what the launches take.  Every call on one GPU shares that GPU's copy, so
+// calls that overlap on one GPU race.  k_cell_residual writes every element
+// residual entry that k_res_gather reads, so nothing is cleared between calls.
+
+namespace {
+constexpr int64_t NOALLOC_DOUBLES = int64_t(1) << 24;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+__global__ void k_fill(int64_t n, double v, double *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = v;
+}
+}  // namespace
+
+int mhd_noalloc_init(mhd_ctx *c) {
+  if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES) {
+    c->err = "device index beyond the allocation-free workspace table";
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    void *p = nullptr;
+    MHD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[c->device] = static_cast<double *>(p);
+  }
+  c->d_scratch = g_scratch_ptr[c->device];
+  return 0;
+}
+
+int mhd_launch_residual_noalloc(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s) {
+  if ((int64_t)c->ncells * NDOF > NOALLOC_DOUBLES) {
+    c->err = "mhd_residual_d_noalloc: " + std::to_string(c->ncells) + " cells exceed the allocation-free workspace (" +
+             std::to_string(NOALLOC_DOUBLES / NDOF) + " cells at most)";
+    return -1;
+  }
+  if (!c->d_scratch) {
+    c->err = "mhd_residual_d_noalloc: workspace address not set up for this device";
+    return -2;
+  }
+  if (c->ncells > 0) {
+    hipLaunchKernelGGL(k_cell_residual, dim3(c->ncells), dim3(WG), 0, s, c->ncells, c->d_xyz, c->d_dofs, c->d_sign,
+                       c->d_dir, d_x, c->d_ref, c->d_scratch, c->phys, 0);
+    hipLaunchKernelGGL(k_res_gather, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, s, c->n, c->d_crow,
+                       c->d_centry, c->d_scratch, d_r);
+  } else if (c->n > 0) {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)std::min<int64_t>((c->n + 255) / 256, 65536)), dim3(256), 0, s, c->n,
+                       0.0, d_r);
+  }
+  MHD_HIP(hipGetLastError());
+  if (c->nfaces > 0) {
+    hipLaunchKernelGGL(k_face_residual, dim3(c->nfaces), dim3(64), 0, s, c->nfaces, c->d_xyz, c->d_dofs, c->d_sign,
+                       c->d_dir, d_x, c->d_fref, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw, d_r, 0);
+    MHD_HIP(hipGetLastError());
+  }
+  return 0;
 }
 
 // val_t[p] = val[tperm[p]]: the CSR values of A^T on the (symmetric) pattern

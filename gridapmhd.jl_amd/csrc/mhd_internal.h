@@ -113,6 +113,7 @@ struct mhd_ctx {
   int32_t max_row = 0;                // longest CSR row
   double *d_ebuf = nullptr;           // [ncells][NENT] element buffer (allocated on first use)
   double *d_rbuf = nullptr;           // [ncells][NDOF] element residuals (allocated on first use)
+  double *d_scratch = nullptr;        // this device's allocation-free residual workspace (not owned)
   // solver workspace (allocated lazily)
   struct Solver *solver = nullptr;
   // coarser level of the [u,j] block solve (owned), see mhd_attach_coarse
@@ -154,6 +155,8 @@ int mhd_launch_spmv(mhd_ctx *c, const double *d_val, const double *d_x, double *
 int mhd_launch_cell_integrals(mhd_ctx *c, const double *d_x, double *d_out, hipStream_t s);
 int mhd_launch_residual_mode(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s, int mode);
 int mhd_launch_transpose_values(mhd_ctx *c, const double *d_val, double *d_val_t, hipStream_t s);
+int mhd_noalloc_init(mhd_ctx *c);
+int mhd_launch_residual_noalloc(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s);
 
 #define MHD_HIP(call)                                                                 \
   do {                                                                                \

@@ -152,6 +152,16 @@ int mhd_residual_d(mhd_ctx *ctx, const double *d_x, double *d_r, void *stream);
 int mhd_jacobian_d(mhd_ctx *ctx, const double *d_x, double *d_nzval_csr, void *stream);
 int mhd_csr_arrays(mhd_ctx *ctx, const int64_t **d_rowptr, const int32_t **d_colind);
 
+/* Allocation-free residual: mhd_residual_d with the element residuals kept in
+ * a fixed device workspace of the library (128 MiB per GPU, held until the
+ * process ends) instead of context memory allocated on first use.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  It calls no
+ * hipMalloc, hipFree or hipMemset.  At most 130,055 cells (16,777,216 / 129
+ * element-residual doubles); larger meshes are rejected with an error.  Every
+ * call on one GPU shares that GPU's workspace, so calls that overlap on one
+ * GPU (any context, any stream) race. */
+int mhd_residual_d_noalloc(mhd_ctx *ctx, const double *d_x, double *d_r, void *stream);
 
 /* y = A x with the Jacobian last assembled into the context (device pointers). */
 int mhd_spmv_d(mhd_ctx *ctx, const double *d_x, double *d_y, void *stream);
