@@ -78,7 +78,7 @@ EXPORTS = {
     "mhd_linear_solve": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(MhdSolverOpts),
                                         ctypes.POINTER(MhdSolveInfo)]),
     "mhd_attach_coarse": (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
-                                         c_int32, c_double, c_int32]),
+                                         c_void_p, c_void_p, c_int32, c_double, c_int32]),
     "mhd_set_pivot_threshold": (ctypes.c_int, [c_void_p, c_double]),
     "mhd_smoother_apply_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhd_smoother_info": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
@@ -251,18 +251,20 @@ class Context:
             setattr(o, k, v)
         return o
 
-    def attach_coarse(self, coarse: "Context", P, inj_u, coarse_iters: int = 0, smoother_weight: float = 0.175,
+    def attach_coarse(self, coarse: "Context", P, R, coarse_iters: int = 0, smoother_weight: float = 0.175,
                       coarse_step: str = "minres"):
-        """Hand `coarse` (and the prolongation P, scipy CSR) to this context.
-        coarse_step: "full" or "minres" (minimal-residual scaling of the
-        coarse correction)."""
+        """Hand `coarse` (with the prolongation P and the state interpolation R,
+        scipy CSR) to this context.  coarse_step: "full" or "minres"
+        (minimal-residual scaling of the coarse correction)."""
         rowptr = np.ascontiguousarray(P.indptr, dtype=np.int64)
         col = np.ascontiguousarray(P.indices, dtype=np.int32)
         val = np.ascontiguousarray(P.data, dtype=np.float64)
-        inj = np.ascontiguousarray(inj_u, dtype=np.int32)
+        rrow = np.ascontiguousarray(R.indptr, dtype=np.int64)
+        rcol = np.ascontiguousarray(R.indices, dtype=np.int32)
+        rval = np.ascontiguousarray(R.data, dtype=np.float64)
         self._check(self.lib.mhd_attach_coarse(self.h, coarse.h, P.shape[0], P.shape[1], _ptr(rowptr), _ptr(col),
-                                               _ptr(val), _ptr(inj), int(coarse_iters), float(smoother_weight),
-                                               COARSE_STEP[coarse_step]),
+                                               _ptr(val), _ptr(rrow), _ptr(rcol), _ptr(rval), int(coarse_iters),
+                                               float(smoother_weight), COARSE_STEP[coarse_step]),
                     "mhd_attach_coarse")
         coarse.h = c_void_p()          # owned by this context now
         self._coarse = coarse
@@ -346,8 +348,8 @@ def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.17
     ctxs = [ctx] + [context_from_spaces(c, phys, device=device) for c in chain[1:]]
     # attach from the coarsest upwards so each attached context already carries its chain
     for lev in range(levels - 1, 0, -1):
-        P, inj = multigrid.prolongation(chain[lev - 1], chain[lev])
-        ctxs[lev - 1].attach_coarse(ctxs[lev], P, inj, coarse_iters, smoother_weight, coarse_step)
+        P, R = multigrid.prolongation(chain[lev - 1], chain[lev])
+        ctxs[lev - 1].attach_coarse(ctxs[lev], P, R, coarse_iters, smoother_weight, coarse_step)
     return ctx
 
 

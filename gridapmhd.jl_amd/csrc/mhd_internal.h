@@ -121,7 +121,7 @@ struct mhd_ctx {
 };
 
 // Link from a level to the next coarser one: prolongation P (n1 x n1c CSR),
-// its transpose, the u-state injection and V-cycle temporaries.
+// its transpose, the u-state interpolation and V-cycle temporaries.
 struct CoarseLink {
   mhd_ctx *ctx = nullptr;          // coarse context (owned)
   int64_t n1 = 0, n1c = 0, nuc = 0;
@@ -129,7 +129,9 @@ struct CoarseLink {
   int32_t *d_Pcol = nullptr, *d_Tcol = nullptr;
   double *d_Pval = nullptr, *d_Tval = nullptr;
   float *d_Pval32 = nullptr, *d_Tval32 = nullptr;
-  int32_t *d_inj = nullptr;        // [nuc] fine u dof of every coarse u dof
+  int64_t *d_Rrow = nullptr;       // state interpolation R (nuc x fine nfree[u], CSR):
+  int32_t *d_Rcol = nullptr;       // the fine velocity at every coarse u node
+  double *d_Rval = nullptr;
   int coarse_iters = 10;           // Schwarz-GMRES steps on the coarsest level; 0: dense inverse
   // coarsest level solved exactly: explicit inverse of its [u,j] block (row-major)
   double *d_Ainv = nullptr;        // [n1c][n1c]

@@ -232,10 +232,17 @@ __global__ void k_dense_a11(int64_t n1, const int64_t *__restrict__ rowptr, cons
 }
 
 // coarse state: u by injection from the fine state, everything else zero
-__global__ void k_inject(int64_t nc, int64_t nuc, const int32_t *__restrict__ inj, const double *__restrict__ xf,
-                         double *__restrict__ xc) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc; i += (int64_t)gridDim.x * blockDim.x)
-    xc[i] = i < nuc ? xf[inj[i]] : 0.0;
+// coarse state: xc_u = R xf_u (the fine velocity at the coarse u nodes), the
+// other coarse fields zero
+__global__ void k_state_restrict(int64_t nc, int64_t nuc, const int64_t *__restrict__ rrow,
+                                 const int32_t *__restrict__ rcol, const double *__restrict__ rval,
+                                 const double *__restrict__ xf, double *__restrict__ xc) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    if (i < nuc)
+      for (int64_t k = rrow[i]; k < rrow[i + 1]; ++k) s += rval[k] * xf[rcol[k]];
+    xc[i] = s;
+  }
 }
 
 __global__ void k_split(int64_t n1, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colind,
@@ -1471,7 +1478,8 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
   mhd_ctx *cc = L->ctx;
   hipStream_t cs = cc->stream;
   cc->phys = c->phys;
-  hipLaunchKernelGGL(k_inject, dim3(nblocks(cc->n)), dim3(TB), 0, cs, cc->n, L->nuc, L->d_inj, d_state, cc->d_x);
+  hipLaunchKernelGGL(k_state_restrict, dim3(nblocks(cc->n)), dim3(TB), 0, cs, cc->n, L->nuc, L->d_Rrow, L->d_Rcol,
+                     L->d_Rval, d_state, cc->d_x);
   MHD_HIP(hipGetLastError());
   if ((rc = mhd_launch_jacobian(cc, cc->d_x, cc->d_val, cs))) {
     c->err = "coarse level: " + cc->err;
@@ -1628,7 +1636,7 @@ void mhd_coarse_free(mhd_ctx *c) {
   if (!c || !c->coarse) return;
   CoarseLink *L = c->coarse;
   void *ptrs[] = {L->d_Prow, L->d_Trow, L->d_Pcol, L->d_Tcol, L->d_Pval, L->d_Tval, L->d_Pval32, L->d_Tval32,
-                  L->d_inj, L->rr, L->rc, L->ec, L->d_Ainv, L->d_LU, L->d_piv, L->d_info, L->d_ab};
+                  L->d_Rrow, L->d_Rcol, L->d_Rval, L->rr, L->rc, L->ec, L->d_Ainv, L->d_LU, L->d_piv, L->d_info, L->d_ab};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (L->ctx) mhd_destroy(L->ctx);
@@ -1639,8 +1647,8 @@ void mhd_coarse_free(mhd_ctx *c) {
 extern "C" {
 
 int mhd_attach_coarse(mhd_ctx *c, mhd_ctx *coarse, int64_t nrows, int64_t ncols, const int64_t *P_rowptr,
-                      const int32_t *P_col, const double *P_val, const int32_t *inj_u, int32_t coarse_iters,
-                      double smoother_weight, int32_t coarse_step) {
+                      const int32_t *P_col, const double *P_val, const int64_t *R_rowptr, const int32_t *R_col,
+                      const double *R_val, int32_t coarse_iters, double smoother_weight, int32_t coarse_step) {
   if (coarse_step != 0 && coarse_step != 1) {
     if (c) c->err = "mhd_attach_coarse: coarse_step must be 0 (full) or 1 (minimal residual)";
     return -1;
@@ -1649,7 +1657,7 @@ int mhd_attach_coarse(mhd_ctx *c, mhd_ctx *coarse, int64_t nrows, int64_t ncols,
     if (c) c->err = "mhd_attach_coarse: smoother_weight must be in (0, 1]";
     return -1;
   }
-  if (!c || !coarse || !P_rowptr || !P_col || !P_val || !inj_u) return -1;
+  if (!c || !coarse || !P_rowptr || !P_col || !P_val || !R_rowptr || !R_col || !R_val) return -1;
   if (c->coarse) {
     c->err = "a coarse level is already attached";
     return -1;
@@ -1666,9 +1674,10 @@ int mhd_attach_coarse(mhd_ctx *c, mhd_ctx *coarse, int64_t nrows, int64_t ncols,
       c->err = "mhd_attach_coarse: P column out of range";
       return -1;
     }
-  for (int64_t i = 0; i < coarse->nfree[0]; ++i)
-    if (inj_u[i] < 0 || inj_u[i] >= c->nfree[0]) {
-      c->err = "mhd_attach_coarse: injection index out of range";
+  const int64_t nnzR = R_rowptr[coarse->nfree[0]];
+  for (int64_t k = 0; k < nnzR; ++k)
+    if (R_col[k] < 0 || R_col[k] >= c->nfree[0]) {
+      c->err = "mhd_attach_coarse: state interpolation column out of range";
       return -1;
     }
   MHD_HIP(hipSetDevice(c->device));
@@ -1714,7 +1723,9 @@ int mhd_attach_coarse(mhd_ctx *c, mhd_ctx *coarse, int64_t nrows, int64_t ncols,
   if (!rc) rc = up((void **)&L->d_Tcol, tcol.data(), sizeof(int32_t) * nnzP);
   if (!rc) rc = up((void **)&L->d_Tval, tval.data(), sizeof(double) * nnzP);
   if (!rc) rc = up((void **)&L->d_Tval32, tv32.data(), sizeof(float) * nnzP);
-  if (!rc) rc = up((void **)&L->d_inj, inj_u, sizeof(int32_t) * L->nuc);
+  if (!rc) rc = up((void **)&L->d_Rrow, R_rowptr, sizeof(int64_t) * (L->nuc + 1));
+  if (!rc) rc = up((void **)&L->d_Rcol, R_col, sizeof(int32_t) * nnzR);
+  if (!rc) rc = up((void **)&L->d_Rval, R_val, sizeof(double) * nnzR);
   L->ctx = coarse;
   c->coarse = L;
   if (rc) return rc;

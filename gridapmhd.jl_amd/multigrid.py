@@ -5,9 +5,13 @@ The hierarchy semi-coarsens x and y by 2 and keeps z, the refinement pattern
 of the reference's GMG hierarchy (``nrefs = (2, 2, 1)``,
 src/Meshers/hunt_mesher.jl:140-155; solver src/Solvers/gmg.jl:323-355).
 Coarse nodes are taken from the fine mesh, so the coarse cells are unions of
-fine cells and the fine spaces contain the coarse ones.  The prolongation
-P (fine [u; j] free dofs x coarse [u; j] free dofs) is the exact embedding:
-every fine dof functional is applied to every coarse basis function.
+fine cells; on meshes of axis-aligned boxes (uniform or graded) the fine
+spaces contain the coarse ones.  The prolongation P (fine [u; j] free dofs x
+coarse [u; j] free dofs) is then the exact embedding: every fine dof
+functional is applied to every coarse basis function, at the coarse reference
+point of the fine dof's physical position (Newton inversion of the coarse
+trilinear map; on non-nested, jittered meshes this interpolates the coarse
+field at the fine dof points).
   * u (Q2, nodal): coarse Q2 basis evaluated at the fine node.
   * j (RT1, nodal flux at Gauss points): the coarse field pushed forward with
     its Piola map, evaluated at the fine dof point, pulled back with the fine
@@ -70,24 +74,51 @@ def rt_basis(xi):
     return out
 
 
+def _trilinear(cell_xyz, xi):
+    """Points and Jacobians of trilinear cell maps: cell_xyz (n, 8, 3), xi (n, 3)."""
+    N, dN = fespaces._q1_shape(xi)                           # (n, 8), (n, 8, 3)
+    x = np.einsum("nv,nvi->ni", N, cell_xyz)
+    J = np.einsum("nvl,nvi->nil", dN, cell_xyz)
+    return x, J
+
+
+def invert_map(cell_xyz, X, xi0=None, iters=30):
+    """Reference coordinates of physical points X (n, 3) in cells cell_xyz
+    (n, 8, 3): Newton iterations on the trilinear map (exact in one step on
+    affine cells)."""
+    xi = np.full(X.shape, 0.5) if xi0 is None else xi0.copy()
+    for _ in range(iters):
+        x, J = _trilinear(cell_xyz, xi)
+        d = np.linalg.solve(J, (X - x)[..., None])[..., 0]
+        xi += d
+        if np.max(np.abs(d)) < 1e-15:
+            break
+    return xi
+
+
 def _locate(coarse: CartesianMesh, fine: CartesianMesh, fine_cells: np.ndarray, fine_xi: np.ndarray):
-    """Coarse cell and coarse reference coordinates of fine reference points
-    (nested meshes: the fine cell (i, j, k) lies in coarse cell (i//2, j//2, k))."""
+    """Coarse cell and coarse reference coordinates of fine reference points.
+    The fine cell (i, j, k) lies in coarse cell (i//2, j//2, k); the coarse
+    reference point is found from the physical point (the inverse of the
+    coarse trilinear map), which is exact on graded axis-aligned meshes and
+    is the natural interpolation on non-nested (jittered) ones."""
     nx, ny, _ = fine.nc
     fi = fine_cells % nx
     fj = (fine_cells // nx) % ny
     fk = fine_cells // (nx * ny)
     cnx, cny = coarse.nc[0], coarse.nc[1]
     cc = fi // 2 + cnx * (fj // 2 + cny * fk)
-    xi = fine_xi.copy()
-    xi[:, 0] = 0.5 * (fine_xi[:, 0] + (fi % 2))
-    xi[:, 1] = 0.5 * (fine_xi[:, 1] + (fj % 2))
+    X, _ = _trilinear(fine.cell_xyz()[fine_cells], fine_xi)
+    guess = fine_xi.copy()
+    guess[:, 0] = 0.5 * (fine_xi[:, 0] + (fi % 2))
+    guess[:, 1] = 0.5 * (fine_xi[:, 1] + (fj % 2))
+    xi = invert_map(coarse.cell_xyz()[cc], X, guess)
     return cc, xi
 
 
-def prolongation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> Tuple[sps.csr_matrix, np.ndarray]:
+def prolongation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> Tuple[sps.csr_matrix, sps.csr_matrix]:
     """P (n1_fine x n1_coarse, CSR) on the [u; j] blocks (badia2024 numbering),
-    and the injection map of coarse free u dofs onto fine free u dofs."""
+    and the state interpolation R (coarse free u x fine free u, CSR)."""
     assert sp_f.style == "badia2024" and sp_c.style == "badia2024"
     fine, coarse = sp_f.mesh, sp_c.mesh
     n1f = sp_f.nfree["u"] + sp_f.nfree["j"]
@@ -140,9 +171,9 @@ def prolongation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> Tuple[sps.csr_
     keep = np.abs(v) > 1e-14
     P = sps.csr_matrix((v[keep], (r[keep], c[keep])), shape=(n1f, n1c))
     P.sum_duplicates()
-    # ---- state injection: coarse free u dof -> fine free u dof at the same node
-    inj = _u_injection(sp_f, sp_c)
-    return P, inj
+    # ---- state transfer: the fine u field at every coarse free u node
+    R = state_interpolation(sp_f, sp_c)
+    return P, R
 
 
 def _jac_at(cell_xyz, xi):
@@ -159,32 +190,52 @@ def _jac_at(cell_xyz, xi):
     return np.einsum("nvl,nvi->nil", dN, cell_xyz)
 
 
-def _u_injection(sp_f, sp_c):
+def state_interpolation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> sps.csr_matrix:
+    """R (coarse free u dofs x fine free u dofs, CSR): the fine Q2 velocity
+    evaluated at the physical position of every coarse u node (the state the
+    coarse Jacobian is assembled at).  A coarse node of a graded mesh that is
+    not a fine node (the midpoint of a coarse edge) gets the fine field there,
+    not the value of the nearest fine node.  Fine Dirichlet values are not
+    part of the free state; the coarse Dirichlet values are the coarse
+    level's own."""
     fine, coarse = sp_f.mesh, sp_c.mesh
-    cdf, cdc = sp_f.cell_dofs.astype(np.int64), sp_c.cell_dofs.astype(np.int64)
-    nfx, nfy = fine.nc[0], fine.nc[1]
+    cdf = sp_f.cell_dofs.astype(np.int64)
+    cdc = sp_c.cell_dofs.astype(np.int64)
+    ids = cdc[:, 0:27]
+    cells, loc = np.nonzero(ids >= 0)
+    g = ids[cells, loc]
+    _, first = np.unique(g, return_index=True)
+    cells, loc = cells[first], loc[first]
+    X, _ = _trilinear(coarse.cell_xyz()[cells], fespaces.Q2_OFFSETS[loc] / 2.0)
+    # the fine cell of the coarse cell (I, J, K) holding X: fine cells (2I + a, 2J + b, K)
     cnx, cny = coarse.nc[0], coarse.nc[1]
-    # coarse cell (I, J, K) corner -> fine cell (2I, 2J, K); coarse node (i, j, k) in {0,1,2}^3 ->
-    # fine lattice offset (2i, 2j, k) from the fine corner cell, i.e. fine cell (2I + i//... )
-    cc = np.arange(coarse.ncells)
-    I, J, K = cc % cnx, (cc // cnx) % cny, cc // (cnx * cny)
-    inj = np.full(sp_c.nfree["u"], -1, dtype=np.int64)
-    for a in range(27):
-        i, j, k = a % 3, (a // 3) % 3, a // 9
-        # fine lattice point (2I*2 + 2i, ...): fine cell index and local node
-        fi = 2 * I + (i if i < 2 else 1)
-        fj = 2 * J + (j if j < 2 else 1)
-        li = 0 if i < 2 else 2
-        lj = 0 if j < 2 else 2
-        fcell = fi + nfx * (fj + nfy * K)
-        lf = li + 3 * lj + 9 * k
-        for comp in range(3):
-            gc = cdc[cc, comp * 27 + a]
-            gf = cdf[fcell, comp * 27 + lf]
-            m = gc >= 0
-            inj[gc[m]] = gf[m]
-    assert (inj >= 0).all()
-    return inj
+    nx, ny = fine.nc[0], fine.nc[1]
+    I, J, K = cells % cnx, (cells // cnx) % cny, cells // (cnx * cny)
+    best_cell = np.zeros(cells.size, dtype=np.int64)
+    best_xi = np.zeros((cells.size, 3))
+    best_out = np.full(cells.size, np.inf)
+    fxyz = fine.cell_xyz()
+    for a in (0, 1):
+        for b in (0, 1):
+            fc = (2 * I + a) + nx * ((2 * J + b) + ny * K)
+            xi = invert_map(fxyz[fc], X)
+            out = np.max(np.maximum(-xi, xi - 1.0), axis=1)
+            take = out < best_out - 1e-12
+            best_cell[take], best_xi[take], best_out[take] = fc[take], xi[take], out[take]
+    N = q2_basis(best_xi)                                    # (n, 27)
+    rows, cols, vals = [], [], []
+    for comp in range(3):
+        gc = cdc[cells, comp * 27 + loc]
+        gf = cdf[best_cell][:, comp * 27:(comp + 1) * 27]
+        m = gf >= 0
+        rows.append(np.broadcast_to(gc[:, None], gf.shape)[m])
+        cols.append(gf[m])
+        vals.append(N[m])
+    r, c, v = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    keep = np.abs(v) > 1e-14
+    R = sps.csr_matrix((v[keep], (r[keep], c[keep])), shape=(sp_c.nfree["u"], sp_f.nfree["u"]))
+    R.sum_duplicates()
+    return R
 
 
 def auto_levels(mesh: CartesianMesh, min_cells: int = 136, min_xy: int = 8) -> int:

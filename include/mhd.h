@@ -196,21 +196,21 @@ int mhd_linear_solve(mhd_ctx *ctx, const double *x_state, const double *rhs, dou
  * src/Meshers/hunt_mesher.jl:140-155, src/Solvers/gmg.jl:323-355).  `coarse` is a
  * context on the coarse mesh with the same parameters and block numbering;
  * P (CSR, nrows = nfree[u]+nfree[j] of ctx, ncols = the same of coarse) maps
- * coarse [u,j] values to fine ones; inj_u[i] is the fine u dof carrying the
- * value of coarse u dof i (state injection for the coarse Jacobian).  ctx takes
- * ownership of coarse (destroyed with ctx; do not destroy it separately).  The
- * [u,j] smoother of mhd_newton_solve then becomes a V-cycle whose vertex-star
- * Schwarz pre/post-smoothing is damped by smoother_weight (0, 1] (the patches
- * overlap up to 8-fold; 0.175 is a good value, 0.25 can diverge).  On the coarsest level
- * coarse_iters Schwarz-GMRES steps are taken, or, with coarse_iters = 0, the
- * level is solved exactly with a dense FP64 inverse (at most 32768 [u,j]
+ * coarse [u,j] values to fine ones; R (CSR, coarse nfree[u] rows x fine
+ * nfree[u] columns) gives the fine velocity at the coarse u nodes, the state
+ * the coarse Jacobian is assembled at.  ctx takes ownership of coarse
+ * (destroyed with ctx; do not destroy it separately).  The [u,j] smoother of
+ * mhd_newton_solve then becomes a V-cycle whose vertex-star Schwarz
+ * pre/post-smoothing is damped by smoother_weight (0, 1] (the patches overlap
+ * up to 8-fold; 0.175 is a good value, 0.25 can diverge).  On the coarsest
+ * level coarse_iters Schwarz-GMRES steps are taken, or, with coarse_iters = 0,
+ * the level is solved exactly with a dense FP64 inverse (at most 32768 [u,j]
  * dofs there).  coarse_step 0 adds the coarse correction with a full step;
  * 1 scales it by the minimal-residual step (robust where the rediscretised
  * coarse level misrepresents the fine one, e.g. high Ha on coarse meshes). */
 int mhd_attach_coarse(mhd_ctx *ctx, mhd_ctx *coarse, int64_t nrows, int64_t ncols, const int64_t *P_rowptr,
-                      const int32_t *P_col, const double *P_val, const int32_t *inj_u, int32_t coarse_iters,
-                      double smoother_weight,
-                      int32_t coarse_step);
+                      const int32_t *P_col, const double *P_val, const int64_t *R_rowptr, const int32_t *R_col,
+                      const double *R_val, int32_t coarse_iters, double smoother_weight, int32_t coarse_step);
 
 /* Preconditioner set-up for the Jacobian last assembled into the context (by
  * mhd_jacobian, a solve, or mhd_jacobian_d with d_nzval_csr = NULL; an error

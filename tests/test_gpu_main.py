@@ -72,3 +72,48 @@ def test_fully_developed_thin_wall_duct():
         if n == 8:
             assert abs(info["uh_0"] / 4.256257 - 1) < 1e-6 and abs(errs[8] / 3.536e-3 - 1) < 1e-3
     assert 5e-3 < errs[16] < 2e-2, errs
+
+
+def test_initial_guesses_through_main():
+    """initial_guess (src/main.jl:285-315) on the GPU: :zero, a Dict of fields
+    interpolated into the spaces, and :solve (a Stokes solve with convection
+    off, then Newton) all reach the same Newton solution of a convective duct;
+    :solve starts Newton closer (smaller first residual).  The walls carry a
+    lid-like tangential velocity (zero on x = +-1 and y = -1, (1 - x^2)/2 along x
+    on y = 1), so the flow is not unidirectional and convection does not vanish."""
+    import gridapmhd_amd as g
+    import numpy as np
+
+    def lid(X):
+        v = np.zeros((X.shape[0], 3))
+        v[:, 0] = 0.25 * (1.0 - X[:, 0] ** 2) * (1.0 + X[:, 1])
+        return v
+
+    def params(x0):
+        model = g.meshers.hunt_generate_base_mesh((8, 8), nz=4, domain=(-1.0, 1.0, -1.0, 1.0, 0.0, 0.3), Ha=20.0,
+                                                  map=lambda x: x)
+        return {"model": model, "x0": x0,
+                "fluid": {"domain": None, "α": 1.0, "β": 1.0 / 20.0, "γ": 400.0 / 20.0, "f": (0.0, 0.0, 1.0),
+                          "B": (0.0, 1.0, 0.0), "ζᵤ": 1.0, "ζⱼ": 1.0, "convection": "newton"},
+                "bcs": {"u": {"tags": "noslip", "values": lid}, "j": {"tags": []},
+                        "thin_wall": [{"domain": "walls", "cw": 0.05, "τ": 100.0}]},
+                "solver": {"solver": "hip", "rtol": 1e-11, "atol": 1e-16, "niter_ls": 2000, "precision": "fp64"}}
+
+    out = {}
+    sols = {}
+    guesses = {"zero": "zero", "solve": "solve",
+               "dict": {"u": (0.0, 0.0, 0.5), "p": 0.0, "j": (0.0, 0.0, 0.0), "φ": 0.0}}
+    for name, x0 in guesses.items():
+        xh, full, o = g.main(params(x0))
+        out[name], sols[name] = o, xh.free_values
+        print(f"[main] initial guess {name}: converged={o['converged']} nls={o['nls_iters']} "
+              f"residuals={['%.2e' % r for r in o['nls_residuals']]}", flush=True)
+        assert o["converged"]
+    sp = xh.spaces
+    ps = sp.field_slices()["p"]
+    for name in ("solve", "dict"):
+        a, b = sols[name].copy(), sols["zero"].copy()
+        a[ps] -= a[ps].mean()
+        b[ps] -= b[ps].mean()
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-9, name
+    assert out["solve"]["nls_residuals"][0] < out["zero"]["nls_residuals"][0]
