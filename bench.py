@@ -96,33 +96,38 @@ def time_events(fn, reps, stream):
     return s.elapsed_time(e) / 1e3 / reps
 
 
-def cpu_baseline(cfg, sp, phys, budget_s=12.0):
-    """Oracle (C restatement, OpenMP on all host cores) Jacobian assembly on a
-    bounded sample of the same C2 cells."""
+def cpu_baseline(sp, phys, ctx, counts, residuals_per_step, log):
+    """CPU price of one FP64 Newton step of the same duct on the host cores:
+    the oracle's element kernels (assembly into the duct's own CSC pattern by
+    binary search) and the solver's sparse, patch, dense and vector kernels,
+    each timed on a bounded sample at the real sizes, times the per-step
+    operation counts of the FP64 GPU solve."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import cases
     import oracle
+    import cpu_baseline as cb
     oracle.build()
-    pb = cases.problem_dict(sp, dict(phys))
-    orc = oracle.Oracle(pb)
-    x = splitmix64_uniform(sp.ndofs, 1234)
-    n = 64
-    t = time.perf_counter()
-    orc.cell_jacobians(x, 0, n)
-    dt = time.perf_counter() - t
-    n2 = int(min(sp.ncells, max(n, n * budget_s / max(dt, 1e-6))))
-    t = time.perf_counter()
-    orc.cell_jacobians(x, 0, n2)
-    dt2 = time.perf_counter() - t
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    try:
-        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
-    except Exception:
-        model = platform.processor()
-    return dict(value=n2 / dt2, unit="cells/s (Jacobian element matrices)", cores=cores, kind="port",
-                sample=f"oracle cell Jacobians of the first {n2} of {sp.ncells} cells of the same duct, "
-                       f"{dt2:.1f} s on {cores} threads ({model})")
+    t0 = time.perf_counter()
+    orc = oracle.Oracle(cases.problem_dict(sp, dict(phys)))
+    colptr, rowval = ctx.pattern()
+    colptr -= 1
+    rowval -= 1
+    n1 = sp.nfree["u"] + sp.nfree["j"]
+    rt = cb.rates(orc, colptr, rowval, n1, sp.ndofs, log=log)
+    del colptr, rowval
+    t, parts = cb.compose(rt, counts, orc.ncells, residuals_per_step)
+    host = cb.host_info()
+    samp = rt["cells_sampled"]
+    return dict(value=round(t, 3), unit="s per Newton step", cores=host["threads"], kind="port",
+                sample=(f"one FP64 Newton step of the same duct priced from CPU kernels timed on samples: element "
+                        f"kernels on {samp['residual']} / {samp['jacobian']} of {samp['of']} cells, sparse "
+                        f"products on the full pattern, 2048 patch products, 512 patch inversions, a 4096^2 dense "
+                        f"product, a 2048^2 LAPACK inverse, 30-vector dot products; times the GPU solve's "
+                        f"operation counts ({time.perf_counter() - t0:.1f} s)"),
+                host=host, isa=rt["isa"], parts={k: round(v, 3) for k, v in parts.items()},
+                rates={k: v for k, v in rt.items() if k not in ("isa", "cells_sampled")},
+                jacobian_cells_per_s=round(1.0 / rt["jacobian_per_cell_s"], 1))
 
 
 def adjoint_config(args):
@@ -366,6 +371,7 @@ def main():
         o = opts(prec, 1e-10, 1e-11, 3000)
         for _ in range(args.warmup):
             ctx.newton_solve(x0, o)
+        ctx.op_counts(reset=True)
         t_tot, n_tot, k_tot = 0.0, 0, 0
         parts = np.zeros(4)
         for _ in range(args.steps):
@@ -378,11 +384,14 @@ def main():
             k_tot += info.krylov_iters
             parts += [info.t_residual, info.t_jacobian, info.t_setup, info.t_solve]
         err = rel_err(xs)
+        counts = {k: v / max(n_tot, 1) for k, v in ctx.op_counts(reset=True).items()}
         results[label] = dict(newton_step_s=t_tot / max(n_tot, 1), solve_s=t_tot / args.steps,
                               newton_iters=n_tot / args.steps,
                               krylov_iters=k_tot / args.steps, rel_err=err, converged=bool(info.converged),
                               phase_s=dict(zip(("residual", "jacobian", "setup", "solve"),
-                                               (parts / max(n_tot, 1)).round(4).tolist())))
+                                               (parts / max(n_tot, 1)).round(4).tolist())),
+                              op_counts_per_step=counts,
+                              residuals_per_step=(n_tot + args.steps) / max(n_tot, 1))
         log(f"{label}: {results[label]}")
 
     extra = {}
@@ -449,8 +458,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("CPU baseline (oracle) ...")
-        cpu = cpu_baseline(cfg, sp, phys)
+        log("CPU baseline (oracle element kernels + CPU solver kernels) ...")
+        cpu = cpu_baseline(sp, phys, ctx, results["fp64"]["op_counts_per_step"],
+                           results["fp64"]["residuals_per_step"], log)
         log(f"cpu: {cpu}")
 
     c2 = args.config == "c2"

@@ -83,7 +83,13 @@ EXPORTS = {
     "mhd_smoother_apply_d": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mhd_smoother_info": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
                                          ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
+    "mhd_op_counts": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int]),
 }
+
+# slots of mhd_op_counts (include/mhd.h)
+OP_COUNT_NAMES = ("spmv_nnz_fp64", "spmv_nnz_fp32", "patch_entries_fp64", "patch_entries_fp32",
+                  "patch_entries_bf16", "dense_entries", "vector_bytes", "setup_cells", "setup_patch_dim3",
+                  "setup_dense_dim3", "outer_iters")
 
 _lib: Optional[ctypes.CDLL] = None
 
@@ -315,6 +321,12 @@ class Context:
         self._check(self.lib.mhd_smoother_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c_),
                                                ctypes.byref(d)), "mhd_smoother_info")
         return (a.value, b.value, c_.value, d.value) if pivoted else (a.value, b.value, c_.value)
+
+    def op_counts(self, reset=False):
+        """Operation counts of the solver work since the last reset (dict, see mhd_op_counts)."""
+        out = np.zeros(len(OP_COUNT_NAMES))
+        self._check(self.lib.mhd_op_counts(self.h, _ptr(out), int(bool(reset))), "mhd_op_counts")
+        return dict(zip(OP_COUNT_NAMES, out.tolist()))
 
     def newton_solve(self, x0, opts: Optional[MhdSolverOpts] = None):
         x = np.array(x0, dtype=np.float64, copy=True)

@@ -132,6 +132,7 @@ struct CoarseLink {
   int64_t *d_Rrow = nullptr;       // state interpolation R (nuc x fine nfree[u], CSR):
   int32_t *d_Rcol = nullptr;       // the fine velocity at every coarse u node
   double *d_Rval = nullptr;
+  int64_t nnzP = 0;                // nonzeros of P (and of P^T)
   int coarse_iters = 10;           // Schwarz-GMRES steps on the coarsest level; 0: dense inverse
   // coarsest level solved exactly: explicit inverse of its [u,j] block (row-major)
   double *d_Ainv = nullptr;        // [n1c][n1c]
@@ -146,6 +147,12 @@ struct CoarseLink {
   int tprec = -1;                  // element type of rr/rc/ec: 0 double, 1 float
 };
 void mhd_coarse_free(mhd_ctx *c);
+
+// Slots of the operation counts (mhd_op_counts)
+enum {
+  OP_SPMV64 = 0, OP_SPMV32, OP_PATCH64, OP_PATCH32, OP_PATCH16, OP_DENSE, OP_VEC,
+  OP_SETUP_CELLS, OP_SETUP_PATCH3, OP_SETUP_DENSE3, OP_OUTER
+};
 void mhd_solver_free(mhd_ctx *c);
 
 // Launchers (mhd_assembly.hip)

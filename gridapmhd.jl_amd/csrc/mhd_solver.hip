@@ -811,9 +811,18 @@ struct Solver {
   double *h_dots = nullptr;  // pinned, 3 * ldots + 1
   int ldots = 0;             // stride of the reduction slots: max(m, kin) + 1
   int nb_dot = 0;
+  // operation counts (mhd_op_counts): nonzeros of the A11 block and of the
+  // A12 / A13 columns of the [u,j] rows, sum of dim^2 and dim^3 over patches
+  int64_t nnz11 = 0, nnz1x = 0;
+  double pent = 0.0, pcube = 0.0;
+  double ops[MHD_NOPS] = {};
 };
 
 namespace {
+
+inline void count(mhd_ctx *c, int k, double v) {
+  if (c->solver) c->solver->ops[k] += v;
+}
 
 // s = gamma / sigma of the adjoint scaling (1 when either vanishes)
 double adjoint_scale(const mhd_ctx *c) {
@@ -972,6 +981,12 @@ int solver_init(mhd_ctx *c, const mhd_solver_opts *o) {
   if ((rc = dmalloc(c, &S->d_Sp, S->h_Mp.size()))) return rc;
   if ((rc = dmalloc(c, &S->d_Sphi, S->h_Mphi.size()))) return rc;
   if ((rc = dmalloc(c, &S->d_split, S->n1))) return rc;
+  for (int64_t i = 0; i < S->n1; ++i) {
+    const int32_t *b = c->h_colind.data() + c->h_rowptr[i], *e = c->h_colind.data() + c->h_rowptr[i + 1];
+    const int64_t k = std::lower_bound(b, e, (int32_t)S->n1) - b;
+    S->nnz11 += k;
+    S->nnz1x += (e - b) - k;
+  }
   hipLaunchKernelGGL(k_split, dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, c->d_rowptr, c->d_colind,
                      S->d_split);
   MHD_HIP(hipGetLastError());
@@ -979,6 +994,10 @@ int solver_init(mhd_ctx *c, const mhd_solver_opts *o) {
   std::vector<int32_t> psize, pidx;
   build_patches(c, psize, pidx, S->PS);
   S->npatch = (int)psize.size();
+  for (int32_t d : psize) {
+    S->pent += (double)d * d;
+    S->pcube += (double)d * d * d;
+  }
   if (S->PS > 256) {
     c->err = "vertex patch larger than 256 dofs";
     return -1;
@@ -1092,6 +1111,7 @@ template <class T>
 int mdot(mhd_ctx *c, const T *V, int64_t ldv, int K, const T *w, int64_t n, double *d_out) {
   Solver *S = c->solver;
   hipLaunchKernelGGL((k_mdot<T>), dim3(S->nb_dot, K), dim3(TB), 0, c->stream, V, ldv, w, n, S->d_part);
+  count(c, OP_VEC, (double)(K + 1) * n * sizeof(T));
   hipLaunchKernelGGL(k_mdot_final, dim3(K), dim3(TB), 0, c->stream, S->nb_dot, S->d_part, d_out);
   MHD_HIP(hipGetLastError());
   return 0;
@@ -1110,6 +1130,7 @@ int arnoldi(mhd_ctx *c, T *V, int64_t ldv, int K, T *w, int64_t n, double *h, do
   if ((rc = mdot(c, V, ldv, K, w, n, d2))) return rc;
   hipLaunchKernelGGL((k_gs_update<T>), dim3(nblocks(n)), dim3(TB), sizeof(double) * K, c->stream, K, V, ldv, d2, w,
                      n);
+  count(c, OP_VEC, 2.0 * (K + 2) * n * sizeof(T) + 2.0 * n * sizeof(T));
   if ((rc = mdot(c, w, 0, 1, w, n, dn))) return rc;
   MHD_HIP(hipMemcpyAsync(S->h_dots, S->d_dots, sizeof(double) * (2 * M + 1), hipMemcpyDeviceToHost, c->stream));
   MHD_HIP(hipStreamSynchronize(c->stream));
@@ -1138,12 +1159,7 @@ int vnorm(mhd_ctx *c, const T *x, int64_t n, double *out) {
 void hess_solve(int k, const std::vector<double> &H, int ldh, double beta, std::vector<double> &y) {
   std::vector<double> R(H), g(k + 1, 0.0);
   g[0] = beta;
-  for (int j = 0; j < k; ++j) {
-    for (int i = 0; i < j; ++i) {
-      // already rotated in place below
-    }
-  }
-  // apply rotations column by column
+  // Givens rotations column by column
   std::vector<double> cs(k), sn(k);
   for (int j = 0; j < k; ++j) {
     for (int i = 0; i < j; ++i) {
@@ -1182,6 +1198,7 @@ int patch_as_on(mhd_ctx *c, const T *r, T *y, int prec, double w, bool accumulat
   else
     hipLaunchKernelGGL((k_patch_apply<__hip_bfloat16, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS,
                        S->d_psize, S->d_pidx, S->d_pperm, S->d_pA16, r, S->d_ppart);
+  count(c, prec == 0 ? OP_PATCH64 : prec == 1 ? OP_PATCH32 : OP_PATCH16, S->pent);
   hipLaunchKernelGGL((k_patch_gather<T>), dim3(nblocks(S->n1)), dim3(TB), 0, st, S->n1, S->d_dptr, S->d_dent,
                      S->d_ppart, w, y, accumulate ? 1 : 0);
   MHD_HIP(hipGetLastError());
@@ -1204,6 +1221,7 @@ int spmv_a11(mhd_ctx *c, const T *x, T *y, int prec) {
   else
     hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 1>), dim3(grid), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
                        S->d_split, c->d_colind, S->d_val32, x, (const T *)nullptr, y);
+  count(c, prec == 0 ? OP_SPMV64 : OP_SPMV32, (double)S->nnz11);
   MHD_HIP(hipGetLastError());
   return 0;
 }
@@ -1220,6 +1238,7 @@ int coarse_direct(mhd_ctx *c, const T *rc, T *ec) {
   const int64_t n = L->n1c;
   hipLaunchKernelGGL((k_dense_rowdot<T>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c->stream, n, L->d_Ainv, rc,
                      ec);
+  count(c, OP_DENSE, (double)n * n);
   MHD_HIP(hipGetLastError());
   return 0;
 }
@@ -1264,6 +1283,8 @@ int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
     hipLaunchKernelGGL((k_csr<float, T, LPR, false>), dim3(gc), dim3(256), 0, c->stream, L->n1c, L->d_Trow, L->d_Tcol,
                        L->d_Tval32, rr, rcv);
   MHD_HIP(hipGetLastError());
+  // residual SpMV, restriction, prolongation, and the second SpMV of either coarse step
+  count(c, prec == 0 ? OP_SPMV64 : OP_SPMV32, 2.0 * S->nnz11 + 2.0 * L->nnzP);
   if (cc->coarse) {
     if ((rc = vcycle<T>(cc, rcv, ec, prec))) return rc;
   } else if (L->coarse_iters == 0) {
@@ -1304,6 +1325,7 @@ int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
   // (rr . w, w . w) on the device; y += a yc, rr -= a w
   if ((rc = mdot(c, rr, ld, 2, wv, S->n1, L->d_ab))) return rc;
   hipLaunchKernelGGL((k_cc_update<T>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, L->d_ab, yc, wv, y, rr);
+  count(c, OP_VEC, 6.0 * S->n1 * sizeof(T));
   MHD_HIP(hipGetLastError());
   // post-smoothing accumulates into y
   if ((rc = patch_as<T>(c, rr, y, prec, L->omega, true))) return rc;
@@ -1345,6 +1367,7 @@ int inner_solve(mhd_ctx *c, int prec) {
   MHD_HIP(hipMemcpyAsync(S->d_y, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL((k_lincomb<T>), dim3(nblocks(n1)), dim3(TB), sizeof(double) * k, c->stream, k, Z1, n1, S->d_y, 0.0,
                      x1, n1);
+  count(c, OP_VEC, (k + 1.0) * n1 * sizeof(T));
   MHD_HIP(hipGetLastError());
   MHD_HIP(hipStreamSynchronize(c->stream));  // y is a pageable host buffer
   return 0;
@@ -1362,12 +1385,14 @@ int precond(mhd_ctx *c, const double *v, double *z, int prec) {
   if (prec == 0) {
     hipLaunchKernelGGL((k_spmv_seg<double, double, double, LPR, 2>), dim3(grid), dim3(256), 0, c->stream, S->n1,
                        c->d_rowptr, S->d_split, c->d_colind, c->d_val, z, v, (double *)S->r1);
+    count(c, OP_SPMV64, (double)S->nnz1x);
     MHD_HIP(hipGetLastError());
     if ((rc = inner_solve<double>(c, 0))) return rc;
     MHD_HIP(hipMemcpyAsync(z, S->x1, sizeof(double) * S->n1, hipMemcpyDeviceToDevice, c->stream));
   } else {
     hipLaunchKernelGGL((k_spmv_seg<double, double, double, LPR, 2>), dim3(grid), dim3(256), 0, c->stream, S->n1,
                        c->d_rowptr, S->d_split, c->d_colind, c->d_val, z, v, S->t1);
+    count(c, OP_SPMV64, (double)S->nnz1x);
     hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, S->t1,
                        (float *)S->r1);
     MHD_HIP(hipGetLastError());
@@ -1425,6 +1450,7 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
   // the patches whose diagonal pivots fail the threshold test
   const int64_t pe = (int64_t)S->PS * S->PS;
   S->nflagged = 0;
+  count(c, OP_SETUP_PATCH3, S->pcube);
   for (int p0 = 0; p0 < S->npatch; p0 += S->pchunk) {
     const int np = std::min(S->pchunk, S->npatch - p0);
     double *mat = S->d_pscratch;
@@ -1485,6 +1511,7 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
     c->err = "coarse level: " + cc->err;
     return rc;
   }
+  count(c, OP_SETUP_CELLS, (double)cc->ncells);
   if (S->transposed) {
     // adjoint: the coarse levels carry A_c^T (same transfers P, P^T); the
     // element buffer, free after the assembly, holds the transposed copy
@@ -1529,6 +1556,7 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
       c->err = "rocsolver LU of the coarsest level failed";
       return -2;
     }
+    count(c, OP_SETUP_DENSE3, (double)m * m * m);
     int32_t info = 0;
     MHD_HIP(hipMemcpyAsync(&info, L->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, cs));
     MHD_HIP(hipStreamSynchronize(cs));
@@ -1564,6 +1592,8 @@ int fgmres(mhd_ctx *c, const double *b, double *x, double tol, int maxit, int pr
     for (int j = 0; j < m && it < maxit; ++j) {
       if ((rc = precond(c, S->V + j * n, S->Z + j * n, prec))) return rc;
       if ((rc = mhd_launch_spmv(c, c->d_val, S->Z + j * n, S->w, c->stream))) return rc;
+      count(c, OP_SPMV64, (double)c->nnz);
+      count(c, OP_OUTER, 1.0);
       double nrm;
       if ((rc = arnoldi<double>(c, S->V, n, j + 1, S->w, n, h.data(), &nrm))) return rc;
       for (int i = 0; i <= j; ++i) H[i * m + j] = h[i];
@@ -1601,6 +1631,8 @@ int fgmres(mhd_ctx *c, const double *b, double *x, double tol, int maxit, int pr
     MHD_HIP(hipGetLastError());
     // true residual for the restart: w = b - J x
     if ((rc = mhd_launch_spmv(c, c->d_val, x, S->w, c->stream))) return rc;
+    count(c, OP_SPMV64, (double)c->nnz);
+    count(c, OP_VEC, (k + 2.0) * n * sizeof(double));
     hipLaunchKernelGGL((k_scale_copy<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, -1.0, S->w, S->w);
     hipLaunchKernelGGL((k_axpy<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, 1.0, b, S->w);
     MHD_HIP(hipGetLastError());
@@ -1685,6 +1717,7 @@ int mhd_attach_coarse(mhd_ctx *c, mhd_ctx *coarse, int64_t nrows, int64_t ncols,
   L->n1 = n1;
   L->n1c = n1c;
   L->nuc = coarse->nfree[0];
+  L->nnzP = nnzP;
   L->coarse_iters = coarse_iters <= 0 ? 0 : std::min(coarse_iters, 120);
   if (L->coarse_iters == 0 && !coarse->coarse && n1c > 32768) {
     c->err = "mhd_attach_coarse: coarse_iters = 0 (dense coarsest solve) needs at most 32768 [u,j] dofs there";
@@ -1738,6 +1771,19 @@ int mhd_attach_coarse(mhd_ctx *c, mhd_ctx *coarse, int64_t nrows, int64_t ncols,
     for (mhd_ctx *q = coarse->coarse->ctx; q; q = q->coarse ? q->coarse->ctx : nullptr) q->stream = c->stream;
   }
   if (c->solver) mhd_solver_free(c);  // re-initialised with the chain on the next solve
+  return 0;
+}
+
+int mhd_op_counts(mhd_ctx *c, double *out, int reset) {
+  if (!c || !out) return -1;
+  for (int k = 0; k < MHD_NOPS; ++k) out[k] = 0.0;
+  for (mhd_ctx *q = c; q; q = q->coarse ? q->coarse->ctx : nullptr) {
+    if (!q->solver) continue;
+    for (int k = 0; k < MHD_NOPS; ++k) {
+      out[k] += q->solver->ops[k];
+      if (reset) q->solver->ops[k] = 0.0;
+    }
+  }
   return 0;
 }
 

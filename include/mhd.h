@@ -242,6 +242,26 @@ int mhd_set_pivot_threshold(mhd_ctx *ctx, double tau);
  * patches of the last set-up inverted with partial pivoting. */
 int mhd_smoother_info(mhd_ctx *ctx, int64_t *npatch, int64_t *patch_dim, int64_t *patch_entries, int64_t *npivoted);
 
+/* Operation counts of the solver work since the last reset, summed over this
+ * context and its coarse levels.  No counterpart in the reference: a
+ * measurement aid that lets other hardware be priced on exactly the work of a
+ * solve (the CPU Newton-step baseline).  out[MHD_NOPS]:
+ *   0  nonzeros streamed by SpMVs with FP64 values (A, the A11 / A1x blocks, P, P^T)
+ *   1  nonzeros streamed by SpMVs with FP32 values
+ *   2  patch-inverse entries applied by the smoother in FP64
+ *   3  ... in FP32
+ *   4  ... in BF16
+ *   5  entries of the dense coarsest inverse applied
+ *   6  bytes of Krylov vector work (dot products, Gram-Schmidt updates,
+ *      linear combinations, minimal-residual coarse steps)
+ *   7  cells of the coarse-level Jacobian assemblies (preconditioner set-up)
+ *   8  sum of dim^3 over the inverted patches (set-up)
+ *   9  sum of n^3 over the dense coarsest inversions (set-up)
+ *   10 outer FGMRES iterations
+ * reset != 0 zeroes the counts after reading them. */
+#define MHD_NOPS 11
+int mhd_op_counts(mhd_ctx *ctx, double *out, int reset);
+
 #ifdef __cplusplus
 }
 #endif

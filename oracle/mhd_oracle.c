@@ -604,6 +604,68 @@ int orc_cell_jacobians(const orc_problem *P, const double *xfree, int c0, int c1
   return 0;
 }
 
+/* Residual assembled over cells [c0, c1): r[gi] += re[a] for the free dofs of
+ * each cell (the vector assembly of src/main.jl:218-233, summed with atomics
+ * across threads). */
+int orc_assemble_residual(const orc_problem *P, const double *xfree, int c0, int c1, double *r) {
+#pragma omp parallel
+  {
+    double re[ND];
+#pragma omp for schedule(dynamic, 4)
+    for (int c = c0; c < c1; ++c) {
+      cell_kernel(P, xfree, c, re, NULL);
+      const int32_t *cd = P->cell_dofs + (size_t)c * ND;
+      for (int a = 0; a < ND; ++a)
+        if (cd[a] >= 0) {
+#pragma omp atomic
+          r[cd[a]] += re[a];
+        }
+    }
+  }
+  return 0;
+}
+
+/* Jacobian assembled over cells [c0, c1) into the values of an existing CSC
+ * pattern (0-based colptr / rowval, rows sorted inside each column): each
+ * touched (test a, trial b) entry is located by binary search in column
+ * cell_dofs[b], the way a sparse assembler adds a cell matrix into a fixed
+ * pattern (src/main.jl:218-233).  Returns the number of entries not found. */
+int64_t orc_assemble_csc(const orc_problem *P, const double *xfree, int c0, int c1, const int64_t *colptr,
+                         const int64_t *rowval, double *nzval) {
+  int64_t missing = 0;
+#pragma omp parallel reduction(+ : missing)
+  {
+    double *Ke = (double *)malloc(sizeof(double) * ND * ND);
+#pragma omp for schedule(dynamic, 1)
+    for (int c = c0; c < c1; ++c) {
+      cell_kernel(P, xfree, c, NULL, Ke);
+      const int32_t *cd = P->cell_dofs + (size_t)c * ND;
+      for (int b = 0; b < ND; ++b) {
+        const int32_t gj = cd[b];
+        if (gj < 0) continue;
+        const int tb = dof_type(b);
+        for (int a = 0; a < ND; ++a) {
+          const int32_t gi = cd[a];
+          if (gi < 0 || !TOUCHED[dof_type(a)][tb]) continue;
+          int64_t lo = colptr[gj], hi = colptr[gj + 1];
+          while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (rowval[mid] < gi) lo = mid + 1; else hi = mid;
+          }
+          if (lo < colptr[gj + 1] && rowval[lo] == gi) {
+#pragma omp atomic
+            nzval[lo] += Ke[a * ND + b];
+          } else {
+            ++missing;
+          }
+        }
+      }
+    }
+    free(Ke);
+  }
+  return missing;
+}
+
 int orc_face_residuals(const orc_problem *P, const double *xfree, double *out) {
 #pragma omp parallel for schedule(dynamic, 4)
   for (int f = 0; f < P->nfaces; ++f) face_kernel(P, xfree, f, out + (size_t)f * NJ, NULL);

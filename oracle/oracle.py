@@ -12,9 +12,13 @@ Layers restated here (reference files read as text):
     duplicates summed, rows sorted inside columns, explicit zeros kept.
   * Newton with a direct solve per step: ``_solver(::Val{:julia})``
     (src/main.jl:181-186) = ``NewtonSolver(LUSolver(); maxiter, rtol=1e-6)``.
-  * FGMRES + upper block-triangular preconditioner of ``Badia2024Solver``
-    (src/Solvers/badia2024.jl:2-48).
+    The iterative ``Badia2024Solver`` (src/Solvers/badia2024.jl:2-48) is not
+    restated: it converges to the solution of the same discrete system, which
+    the direct solve fixes.
   * Post-processing norms of ``_hunt`` (src/Applications/hunt.jl:247-260).
+
+oracle/cpu_baseline.py + cpu_step.c price one Newton step on the host CPU
+(timing only, for the benchmark's CPU baseline).
 
 The direct solve uses scipy's SuperLU.  A singular Jacobian (constant pressure
 mode of a fully Dirichlet velocity; constant potential when every wall is

@@ -97,7 +97,7 @@ def test_initial_guesses_through_main():
                           "B": (0.0, 1.0, 0.0), "ζᵤ": 1.0, "ζⱼ": 1.0, "convection": "newton"},
                 "bcs": {"u": {"tags": "noslip", "values": lid}, "j": {"tags": []},
                         "thin_wall": [{"domain": "walls", "cw": 0.05, "τ": 100.0}]},
-                "solver": {"solver": "hip", "rtol": 1e-11, "atol": 1e-16, "niter_ls": 2000, "precision": "fp64"}}
+                "solver": {"solver": "hip", "rtol": 1e-16, "atol": 1e-12, "niter_ls": 2000, "precision": "fp64"}}
 
     out = {}
     sols = {}
