@@ -353,14 +353,19 @@ def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.17
     if levels <= 1:
         return ctx
     bc = sp.bc_spec
-    chain = [sp]
-    for _ in range(levels - 1):
+    # the coarse spaces and transfers depend on the mesh only: kept on the
+    # fine spaces for further contexts of the same problem
+    cache = sp.__dict__.setdefault("_multigrid_cache", {"chain": [sp], "transfers": {}})
+    chain = cache["chain"]
+    while len(chain) < levels:
         mesh = multigrid.coarsen_mesh(chain[-1].mesh)
         chain.append(fespaces.setup_fe_spaces(mesh, bc["u_tags"], bc["j_tags"], bc["thin_walls"], style=sp.style))
-    ctxs = [ctx] + [context_from_spaces(c, phys, device=device) for c in chain[1:]]
+    ctxs = [ctx] + [context_from_spaces(c, phys, device=device) for c in chain[1:levels]]
     # attach from the coarsest upwards so each attached context already carries its chain
     for lev in range(levels - 1, 0, -1):
-        P, R = multigrid.prolongation(chain[lev - 1], chain[lev])
+        if lev not in cache["transfers"]:
+            cache["transfers"][lev] = multigrid.prolongation(chain[lev - 1], chain[lev])
+        P, R = cache["transfers"][lev]
         ctxs[lev - 1].attach_coarse(ctxs[lev], P, R, coarse_iters, smoother_weight, coarse_step)
     return ctx
 

@@ -74,24 +74,56 @@ def rt_basis(xi):
     return out
 
 
-def _trilinear(cell_xyz, xi):
-    """Points and Jacobians of trilinear cell maps: cell_xyz (n, 8, 3), xi (n, 3)."""
-    N, dN = fespaces._q1_shape(xi)                           # (n, 8), (n, 8, 3)
-    x = np.einsum("nv,nvi->ni", N, cell_xyz)
-    J = np.einsum("nvl,nvi->nil", dN, cell_xyz)
-    return x, J
+def _trilinear(cell_xyz, xi, points=True):
+    """Points and Jacobians J[n, i, l] = dx_i / dxi_l of trilinear cell maps:
+    cell_xyz (n, 8, 3), xi (n, 3).  Vertex v = bx + 2 by + 4 bz."""
+    C = np.ascontiguousarray(cell_xyz.transpose(1, 2, 0))   # (8, 3, n)
+    t = [(1.0 - xi[:, a], np.ascontiguousarray(xi[:, a])) for a in range(3)]
+    n = xi.shape[0]
+    x = np.zeros((3, n)) if points else None
+    J = np.zeros((3, 3, n))
+    for v in range(8):
+        bx, by, bz = v & 1, (v >> 1) & 1, (v >> 2) & 1
+        fx, fy, fz = t[0][bx], t[1][by], t[2][bz]
+        fyz, fxz, fxy = fy * fz, fx * fz, fx * fy
+        if not bx:
+            fyz = -fyz
+        if not by:
+            fxz = -fxz
+        if not bz:
+            fxy = -fxy
+        X = C[v]
+        if points:
+            x += (fx * (fy * fz)) * X
+        J[:, 0] += fyz * X
+        J[:, 1] += fxz * X
+        J[:, 2] += fxy * X
+    return (x.T if points else None), J.transpose(2, 0, 1)
 
 
-def invert_map(cell_xyz, X, xi0=None, iters=30):
+def _solve3(J, r):
+    """J^-1 r for batches of 3 x 3 matrices (rows of the inverse = cross
+    products of the columns over the determinant)."""
+    c0, c1, c2 = J[:, :, 0], J[:, :, 1], J[:, :, 2]
+    k0, k1, k2 = np.cross(c1, c2), np.cross(c2, c0), np.cross(c0, c1)
+    det = np.einsum("ni,ni->n", c0, k0)
+    return np.stack([np.einsum("ni,ni->n", k0, r), np.einsum("ni,ni->n", k1, r),
+                     np.einsum("ni,ni->n", k2, r)], 1) / det[:, None]
+
+
+def invert_map(cell_xyz, X, xi0=None, iters=30, tol=1e-12):
     """Reference coordinates of physical points X (n, 3) in cells cell_xyz
-    (n, 8, 3): Newton iterations on the trilinear map (exact in one step on
-    affine cells)."""
+    (n, 8, 3): Newton iterations on the trilinear map, each point until its
+    update is below `tol` (the error after that update is of order tol^2;
+    affine cells converge in one step)."""
     xi = np.full(X.shape, 0.5) if xi0 is None else xi0.copy()
+    act = np.arange(X.shape[0])
     for _ in range(iters):
-        x, J = _trilinear(cell_xyz, xi)
-        d = np.linalg.solve(J, (X - x)[..., None])[..., 0]
-        xi += d
-        if np.max(np.abs(d)) < 1e-15:
+        x, J = _trilinear(cell_xyz[act], xi[act])
+        d = _solve3(J, X[act] - x)
+        xi[act] += d
+        act = act[np.max(np.abs(d), axis=1) >= tol]
+        if act.size == 0:
             break
     return xi
 
@@ -156,10 +188,10 @@ def prolongation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> Tuple[sps.csr_
     Jc = _jac_at(coarse.cell_xyz()[cc], xi_c)
     detf, detc = np.linalg.det(Jf), np.linalg.det(Jc)
     # fine dof = axis comp of det_f Jf^-1 (Jc psi_c / det_c)
-    Mmat = np.einsum("n,nla,nab,n->nlb", detf, np.linalg.inv(Jf), Jc, 1.0 / detc)   # (n, 3, 3)
+    Mmat = np.matmul(np.linalg.inv(Jf), Jc) * (detf / detc)[:, None, None]         # (n, 3, 3)
     psi = rt_basis(xi_c)                                                             # (n, 36, 3)
     row_axis = Mmat[np.arange(len(cells)), axis[loc], :]                             # (n, 3)
-    W = np.einsum("nb,nrb->nr", row_axis, psi)                                       # (n, 36)
+    W = (psi * row_axis[:, None, :]).sum(-1)                                         # (n, 36)
     sgn_c = np.ones((len(cells), 36))
     sgn_c[:, :24] = np.repeat(sp_c.rt_sign[cc].astype(float), 4, axis=1)
     sgn_f = np.where(loc < 24, sp_f.rt_sign[cells, np.minimum(loc, 23) // 4].astype(float), 1.0)
@@ -178,16 +210,7 @@ def prolongation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> Tuple[sps.csr_
 
 def _jac_at(cell_xyz, xi):
     """Jacobians of the trilinear maps of cells[k] at points xi[k] -> (n, 3, 3)."""
-    n = xi.shape[0]
-    dN = np.empty((n, 8, 3))
-    for v in range(8):
-        b = [(v >> a) & 1 for a in range(3)]
-        f = [xi[:, a] if b[a] else 1.0 - xi[:, a] for a in range(3)]
-        df = [np.full(n, 1.0) if b[a] else np.full(n, -1.0) for a in range(3)]
-        dN[:, v, 0] = df[0] * f[1] * f[2]
-        dN[:, v, 1] = f[0] * df[1] * f[2]
-        dN[:, v, 2] = f[0] * f[1] * df[2]
-    return np.einsum("nvl,nvi->nil", dN, cell_xyz)
+    return _trilinear(cell_xyz, xi, points=False)[1]
 
 
 def state_interpolation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> sps.csr_matrix:

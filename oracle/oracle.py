@@ -257,14 +257,23 @@ class Oracle:
 # ---------------------------------------------------------------------------
 # Solvers
 
-def null_pins(nfields: Dict[str, int], pin_p=True, pin_phi=False):
-    """Indices of the dofs pinned to remove constant-pressure/potential modes."""
-    off = 0
+def null_pins(spaces, pin_p=True, pin_phi=False):
+    """Indices of the dofs pinned to remove constant-pressure/potential modes:
+    the first free dof of p (and of phi).  ``spaces`` is a Spaces object (any
+    multifield numbering, read from its field slices) or a dict of field sizes
+    in the consecutive (u, p, j, phi) numbering."""
+    if hasattr(spaces, "field_slices"):
+        first = {f: s.start for f, s in spaces.field_slices().items()}
+    else:
+        first, off = {}, 0
+        for f in ("u", "p", "j", "phi"):
+            first[f] = off
+            off += spaces[f]
     out = []
-    for f in ("u", "p", "j", "phi"):
-        if (f == "p" and pin_p) or (f == "phi" and pin_phi):
-            out.append(off)
-        off += nfields[f]
+    if pin_p:
+        out.append(first["p"])
+    if pin_phi:
+        out.append(first["phi"])
     return out
 
 

@@ -62,7 +62,7 @@ def newton(orc, x, pins=(), rtol=1e-13, atol=1e-15, maxiter=25):
 def make_c2_params():
     sp, phys = cases.c2_params_case()
     orc = oracle.Oracle(cases.problem_dict(sp, phys))
-    x, hist = newton(orc, np.zeros(sp.ndofs), pins=oracle.null_pins(sp.nfree))
+    x, hist = newton(orc, np.zeros(sp.ndofs), pins=oracle.null_pins(sp))
     meta = dict(case="c2_params_8x8x8", **{k: v for k, v in cases.C2_PARAMS.items()}, zeta=0.0,
                 newton_residuals=hist, ndofs=sp.ndofs)
     return dict(x=x, meta=np.array(json.dumps(meta)))

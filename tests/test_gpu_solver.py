@@ -48,7 +48,7 @@ def test_newton_solve_matches_direct(name, precision, orc_mod):
     sp, phys = cases.duct_thin_wall_case(style="badia2024", **kw)
     # oracle: plain method (zeta = 0), Newton + direct solve
     orc = orc_mod.Oracle(cases.problem_dict(sp, phys))
-    pins = orc_mod.null_pins(sp.nfree)
+    pins = orc_mod.null_pins(sp)
     x_ref = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), maxiter=20, rtol=1e-13, atol=1e-15, pins=pins)
     gphys = dict(phys, zeta_u=1.0, zeta_j=1.0)
     ctx = g._lib.context_from_spaces(sp, gphys)
@@ -76,7 +76,7 @@ def test_multilevel_solve_matches_direct(nc, levels, coarse_iters, step, precisi
     sp, phys = cases.duct_thin_wall_case(nc, Ha=30.0, Re=10.0, cw=0.02, convection="newton", style="badia2024")
     orc = orc_mod.Oracle(cases.problem_dict(sp, phys))
     x_ref = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), maxiter=20, rtol=1e-13, atol=1e-15,
-                                  pins=orc_mod.null_pins(sp.nfree))
+                                  pins=orc_mod.null_pins(sp))
     ctx = g._lib.context_with_levels(sp, dict(phys, zeta_u=1.0, zeta_j=1.0), levels=levels,
                                      coarse_iters=coarse_iters, coarse_step=step)
     try:
@@ -103,6 +103,7 @@ FEATURE_CASES = {
 
 
 _FEATURE_REF = {}
+_FEATURE_REPRO = {}
 
 
 @pytest.mark.parametrize("precision", [0, 1])
@@ -129,11 +130,18 @@ def test_feature_cases_match_direct(name, precision, orc_mod):
     # On non-affine cells Gauss quadrature of div v is inexact, so a constant
     # pressure is no longer an exact null mode: the system is nonsingular and
     # pinning a pressure dof would drop an equation that does not hold.
-    pins = () if jitter else orc_mod.null_pins(sp.nfree)
+    pins = () if jitter else orc_mod.null_pins(sp)
     if name not in _FEATURE_REF:
         _FEATURE_REF[name] = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), maxiter=20, rtol=1e-13, atol=1e-15,
                                                    pins=pins)
     x_ref = _FEATURE_REF[name]
+    if kw["Ha"] >= 1000.0 and name not in _FEATURE_REPRO:
+        # the oracle's own Newton restarted from its solution perturbed by 1e-9
+        xp = x_ref + 1e-9 * np.linalg.norm(x_ref) / np.sqrt(x_ref.size) * \
+            np.random.default_rng(1).standard_normal(x_ref.size)
+        x2 = orc_mod.newton_direct(orc, xp, maxiter=20, rtol=1e-13, atol=1e-15, pins=pins)
+        _FEATURE_REPRO[name] = cases.field_errors(sp, x2, x_ref)
+        _say(f"  oracle restarted from a 1e-9 perturbation moves by {_FEATURE_REPRO[name]}")
     ctx = g._lib.context_with_levels(sp, dict(phys, zeta_u=1.0, zeta_j=1.0), levels=2, coarse_iters=0,
                                      coarse_step="minres")
     try:
@@ -142,7 +150,7 @@ def test_feature_cases_match_direct(name, precision, orc_mod):
         # Newton target is 1e-14 with cheap 1e-6 linear solves per step.
         o = _opts(g, precision, rtol=1e-14, ls_rtol=1e-6)
         o.atol = 1e-18
-        o.max_newton = 6
+        o.max_newton = 10
         o.inner_iters = 10
         o.restart = 300
         o.max_krylov = 600
@@ -155,15 +163,18 @@ def test_feature_cases_match_direct(name, precision, orc_mod):
         assert rc >= 0 and relres < 1e-12
         bar = 1e-10 if precision == 0 else 1e-8
         if kw["Ha"] >= 1000.0:
-            # Ha = 1000 on 8 x 8 x 3 stretched cells is ill-conditioned beyond
-            # FP64: the oracle's own Newton, restarted from its solution
-            # perturbed by 1e-9, moves by 1.5e-5 of |x| (its pressure by 2x,
-            # u and j by 1e-6).  Bars: u, j, phi 1e-6; p 1e-4 of |x|.
-            ps = sp.field_slices()["p"]
-            a, b = x[ps] - x[ps].mean(), x_ref[ps] - x_ref[ps].mean()
-            assert np.linalg.norm(a - b) / np.linalg.norm(x_ref) < 1e-4, err
+            # Ha = 1000 on 8 x 8 x 3 stretched cells: cond_1 of the pinned
+            # Jacobian is about 1e13 (test_oracle.py), so the FP64 solution
+            # itself is determined only to what the oracle's own Newton,
+            # restarted from a 1e-9 perturbation, reproduces: u, j, phi to
+            # 1e-10 .. 2e-10, the pressure to 3e-3 of |p| (3e-9 of |x|).
+            # u, j, phi: the standard bar, or ten times that reproducibility
+            # where it is larger.  p: 1e-6 of |x| (measured: 1e-8 FP64,
+            # 7e-8 mixed, both at a 5e-15 relative residual).
+            rep = _FEATURE_REPRO[name]
             for f in ("u", "j", "phi"):
-                assert err[f] < 1e-6, err
+                assert err[f] < max(bar, 10.0 * rep[f]), (f, err, rep)
+            assert err["all"] < 1e-6, (err, rep)
         else:
             assert err["all"] < bar, err
     finally:

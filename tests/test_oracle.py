@@ -42,7 +42,7 @@ def test_hunt_norms_match_reference(cx, Ha, orc_mod):
     assert (sp.nfree["u"], sp.nfree["p"], sp.nfree["j"], sp.nfree["phi"]) == (
         r["ndofs_u"], r["ndofs_p"], r["ndofs_j"], r["ndofs_phi"])
     orc = orc_mod.Oracle(cases.problem_dict(sp, phys))
-    x = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), pins=orc_mod.null_pins(sp.nfree))
+    x = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), pins=orc_mod.null_pins(sp))
     n = orc.cell_norms(x, nq=4).sum(0)
     uh_l2, uh_h1 = np.sqrt(n[0]), np.sqrt(n[0] + n[1])
     jh_l2 = Ha * np.sqrt(n[2])  # jh = sigma u0 B0 jbar with B0 = Ha (src/Applications/hunt.jl:217)
@@ -217,7 +217,7 @@ def test_hunt_analytic_errors_match_reference(cx, Ha, orc_mod):
     sp, phys = cases.hunt_case(cx, Ha=Ha)
     r = _gold(cx, Ha)[0]
     orc = orc_mod.Oracle(cases.problem_dict(sp, phys))
-    x = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), pins=orc_mod.null_pins(sp.nfree))
+    x = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), pins=orc_mod.null_pins(sp))
     e = postprocess.hunt_errors(sp, x, 1.0, 1.0, 1.0, -1.0, Ha, r["nsums"], u_scale=1.0, j_scale=Ha)
     for name in ("eu_h1", "ej_l2"):
         rel = abs(e[name] / r[name] - 1.0)
@@ -247,8 +247,41 @@ def test_thin_wall_duct_converges_to_general_hunt(orc_mod):
         sp = main_mod.setup_fe_spaces(p)
         orc = orc_mod.Oracle(cases.problem_dict(sp, fluid_physics(p["fluid"])))
         x = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), maxiter=5, rtol=1e-12, atol=1e-15,
-                                  pins=orc_mod.null_pins(sp.nfree, pin_phi=(cw == 0.0)))
+                                  pins=orc_mod.null_pins(sp, pin_phi=(cw == 0.0)))
         return postprocess.general_hunt_error(sp, x, 1.0, cw_model, -1.0, 10.0, 50)
     e4, e8 = err((4, 4), 0.1, 0.1), err((8, 8), 0.1, 0.1)
     assert e8 < 0.5 * e4 and e8 < 2e-2, (e4, e8)
     assert err((4, 4), 0.1, 0.0) > 0.2
+
+
+def test_pinned_jacobian_condition_at_ha1000(orc_mod):
+    """Condition of the oracle's pinned Jacobian at the solution of the
+    Ha = 1000, tanh(beta = 3) thin-wall duct on 8 x 8 x 3 cells (block
+    numbering): cond_1 = |A|_1 |A^-1|_1 with |A^-1|_1 from scipy's onenormest
+    on the LU solve.  With the first pressure dof pinned it is about 1e13
+    (so a 1e-16 residual bounds the error only to about 1e-3, and the
+    solution is reproducible to about 3e-9 of |x| in practice).  Pinning the
+    first dof of the wrong field leaves the constant-pressure mode: cond_1
+    then exceeds 1e17, which is what a numbering-blind pin did."""
+    import scipy.sparse.linalg as spla
+    import gridapmhd_amd as g
+    sp, phys = cases.duct_thin_wall_case((8, 8, 3), 1000.0, 10.0, 0.05, map=g.meshers.tanh_map(3.0),
+                                         style="badia2024")
+    orc = orc_mod.Oracle(cases.problem_dict(sp, dict(phys)))
+    pins = orc_mod.null_pins(sp)
+    assert pins == [sp.field_slices()["p"].start]
+    x = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), maxiter=20, rtol=1e-13, atol=1e-15, pins=pins)
+    A = orc.jacobian(x).tocsc()
+
+    def cond1(pin):
+        keep = np.ones(sp.ndofs, bool)
+        keep[pin] = False
+        Ak = A[keep][:, keep].tocsc()
+        lu = spla.splu(Ak)
+        n = Ak.shape[0]
+        inv = spla.LinearOperator((n, n), matvec=lu.solve, rmatvec=lambda v: lu.solve(v, trans="T"))
+        return spla.norm(Ak, 1) * spla.onenormest(inv)
+
+    c = cond1(pins[0])
+    assert 1e11 < c < 1e14, c
+    assert cond1(sp.field_slices()["j"].start) > 1e17
