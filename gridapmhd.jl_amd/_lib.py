@@ -347,6 +347,11 @@ def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.17
     from the mesh; `coarse_iters = 0` solves the coarsest level exactly;
     `coarse_step` is "full" or "minres" (see mhd_attach_coarse)."""
     from . import fespaces, multigrid
+    if getattr(sp.mesh, "nc", None) is None:
+        # unstructured (gmsh) meshes have no semi-coarsened hierarchy here
+        if levels not in ("auto", 1):
+            raise ValueError("the [u,j] V-cycle needs a Cartesian mesh hierarchy; use levels=1 on unstructured meshes")
+        levels = 1
     if levels == "auto":
         levels = multigrid.auto_levels(sp.mesh)
     ctx = context_from_spaces(sp, phys, device=device)

@@ -8,14 +8,17 @@ u = u0 ubar, j = sigma u0 B0 jbar) and the errors eu_h1, eu_l2, ej_l2 against
 the analytic solution (``analytic.analytical_hunt_u/j``).  ``thin_wall_duct`` is the fully developed
 duct with thin conducting walls on all four walls, the pattern of
 src/Applications/FullyDevelopedFlow.jl:96-194 used by the benchmarks.
+``expansion`` mirrors ``_expansion`` (src/Applications/expansion.jl:40-258)
+on its gmsh hex meshes.
 """
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 
-from . import analytic, meshers, postprocess
+from . import analytic, mesh_io, meshers, postprocess
 from .main import main
 
 
@@ -152,4 +155,93 @@ def fully_developed(nc=(3, 3), Ha=10.0, dir_B=(0.0, 1.0, 0.0), b=1.0, cw_Ha=0.0,
                 ndofs_j=sp.nfree["j"], ndofs_φ=sp.nfree["phi"])
     if cw_s == 0.0:
         info["e_u_l2_rel"] = postprocess.general_hunt_error(sp, xh.free_values, b, cw_Ha, -1.0, Ha, nsums)
+    return info, xh
+
+
+def setup_expansion_mesh_tags(model) -> None:
+    """``setup_expansion_mesh_tags!`` (src/Applications/expansion.jl:357-380)."""
+    tags = set(model.groups)
+    solid = {"wall_interior", "wall_exterior", "wall_volume"} <= tags
+    if "wall" not in tags:
+        if not solid:
+            raise ValueError("the mesh has neither a 'wall' tag nor the solid wall tags")
+        model.add_tag_from_tags("wall", ["wall_interior", "wall_exterior", "wall_volume"])
+    if "boundary" not in tags:
+        model.add_tag_from_tags("boundary", ["inlet", "outlet", "wall_exterior"] if solid else
+                                ["inlet", "outlet", "wall"])
+    if "interior" not in tags:
+        model.add_tag_from_tags("interior", ["fluid", "wall_volume", "wall_interior"] if solid else ["fluid"])
+
+
+def u_inlet(inlet, Ha, Z, b):
+    """Inlet profiles with mean velocity 1 in the outlet channel
+    (src/Applications/expansion.jl:383-408)."""
+    if inlet == "parabolic":
+        return lambda X: np.stack([36.0 * Z * (X[:, 1] - 1 / Z) * (X[:, 1] + 1 / Z) * (X[:, 2] - b * Z) *
+                                   (X[:, 2] + b * Z), 0 * X[:, 0], 0 * X[:, 0]], 1)
+    if inlet == "shercliff":
+        kp = analytic.kp_shercliff_cartesian(b * Z, Ha / Z)
+
+        def u(X):
+            # the duct's own frame: axis x -> z, y scaled by Z, z scaled by Z
+            Y = np.stack([X[:, 2] * Z, X[:, 1] * Z, X[:, 0]], 1)
+            ux = analytic.analytical_general_hunt_u(b * Z, 0.0, -Z * kp, Ha / Z, 200, Y)[:, 2]
+            ux = np.where((np.abs(Y[:, 0]) > b * Z) | (np.abs(Y[:, 1]) > 1.0), 0.0, ux)
+            return np.stack([ux, 0 * ux, 0 * ux], 1)
+        return u
+    if inlet == "constant":
+        return lambda X: np.tile([Z, 0.0, 0.0], (X.shape[0], 1))
+    raise ValueError(f"unknown inlet {inlet!r}")
+
+
+def expansion_params(mesh="710", mesh_dir=None, solver="hip", formulation="mhd", Z=4.0, b=0.2, N=1.0, Ha=1.0,
+                     cw=0.028, τ=100.0, ζᵤ=0.0, ζⱼ=0.0, inlet="parabolic", initial_value="zero",
+                     solid_coupling="none", niter=None, convection="newton", debug=False):
+    """The ``params`` of ``_expansion`` (src/Applications/expansion.jl:40-220)
+    for ``main``: gmsh mesh and tags, non-dimensionalisation, inlet profile,
+    wall and electric boundary conditions, initial guess."""
+    if solid_coupling not in ("none", "thin_wall"):
+        raise NotImplementedError(f"solid_coupling={solid_coupling!r}: solid regions are not built")
+    if formulation not in ("cfd", "mhd"):
+        raise ValueError("Unknown formulation")
+    msh = mesh if os.path.exists(str(mesh)) else os.path.join(mesh_dir or ".", f"Expansion_{mesh}.msh")
+    model = mesh_io.read_gmsh(msh)
+    setup_expansion_mesh_tags(model)
+    Re = Ha ** 2 / N
+    α, β, γ = (1.0, 1.0 / Re, N) if formulation == "cfd" else (1.0 / N, 1.0 / Ha ** 2, 1.0)
+    u_in = u_inlet(inlet, Ha, Z, b)
+    zero = (0.0, 0.0, 0.0)
+    bcs = {"u": {"tags": ["inlet", "wall"], "values": [u_in, zero]}}
+    if solid_coupling == "none":
+        bcs["j"] = {"tags": ["wall", "inlet", "outlet"], "values": [zero, zero, zero]}
+    else:
+        bcs["j"] = {"tags": ["inlet", "outlet"], "values": [zero, zero]}
+        bcs["thin_wall"] = {"τ": τ, "cw": cw, "domain": ["wall"]}
+    x0 = {"u": u_in, "j": zero, "p": 0.0, "φ": 0.0} if initial_value == "inlet" else initial_value
+    solver = dict(solver) if isinstance(solver, dict) else solver
+    if niter is not None:
+        solver = {"solver": solver} if isinstance(solver, str) else solver
+        solver["niter"] = niter
+    return {
+        "debug": debug, "model": model, "x0": x0, "bcs": bcs, "solver": solver,
+        "fluid": {"domain": "fluid", "α": α, "β": β, "γ": γ, "f": zero, "B": (0.0, 1.0, 0.0), "ζᵤ": ζᵤ, "ζⱼ": ζⱼ,
+                  "convection": convection},
+    }
+
+
+def expansion(mesh="710", mesh_dir=None, title="Expansion", vtk=False, path=".", **kw):
+    """``expansion`` / ``_expansion`` (src/Applications/expansion.jl:1-258) on
+    a gmsh 4.1 hex mesh: ``mesh`` is a file path, or the suffix of
+    ``Expansion_<mesh>.msh`` in ``mesh_dir``; the other keywords are those of
+    ``expansion_params``.  Returns (info, xh)."""
+    params = expansion_params(mesh=mesh, mesh_dir=mesh_dir, **kw)
+    Ha, N = kw.get("Ha", 1.0), kw.get("N", 1.0)
+    Re = Ha ** 2 / N
+    info = {}
+    xh, fullparams, out = main(params, output=info)
+    if vtk:
+        info["vtk"] = postprocess.write_vtk(os.path.join(path, title), xh)
+    sp = xh.spaces
+    info.update(ncells=sp.ncells, ndofs=sp.ndofs, ndofs_u=sp.nfree["u"], ndofs_p=sp.nfree["p"],
+                ndofs_j=sp.nfree["j"], ndofs_φ=sp.nfree["phi"], Ha=Ha, N=N, Re=Re, mesh=str(mesh), title=title)
     return info, xh

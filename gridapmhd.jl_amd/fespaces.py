@@ -73,6 +73,8 @@ class Spaces:
     style: str = "consecutive"
     bc_spec: dict = None           # the u / j Dirichlet tags and thin walls these tables were built from
     thin_wall_faces: List[dict] = dataclasses.field(default_factory=list)
+    node_tagger: Optional[Callable] = None   # tags -> bool mask over the Q2 nodes
+    face_tagger: Optional[Callable] = None   # tags -> bool mask over the faces
 
     @property
     def ncells(self) -> int:
@@ -105,7 +107,10 @@ def setup_fe_spaces(mesh: CartesianMesh, u_tags, j_tags, thin_walls: Sequence[di
     box entity ids); ``thin_walls`` is a list of dicts with ``domain`` (tags),
     ``cw``, ``tau``, ``jw`` (src/parameters.jl:938-947).  ``style`` is the
     multifield numbering (src/fespaces.jl:4-9): "consecutive" = [u; p; j; phi],
-    "badia2024" = [u; j], p, phi."""
+    "badia2024" = [u; j], p, phi.  ``mesh`` is a CartesianMesh or an
+    unstructured ``mesh_io.HexMesh`` (gmsh)."""
+    if getattr(mesh, "nc", None) is None:
+        return _setup_hex_spaces(mesh, u_tags, j_tags, thin_walls, style)
     nx, ny, nz = mesh.nc
     ncells = mesh.ncells
     LX, LY, LZ = mesh.lattice_shape()
@@ -207,6 +212,8 @@ def setup_fe_spaces(mesh: CartesianMesh, u_tags, j_tags, thin_walls: Sequence[di
     assert cd.max() < 2**31
 
     sp = Spaces(
+        node_tagger=lambda tags: mesh.tagged(tags, lat[:, 0], lat[:, 1], lat[:, 2]),
+        face_tagger=lambda tags: mesh.tagged(tags, face_lat[:, 0], face_lat[:, 1], face_lat[:, 2]),
         mesh=mesh,
         nfree={"u": n_u_free, "p": n_p, "j": n_j_free, "phi": n_phi},
         ndir={"u": n_u_dir, "j": n_j_dir},
@@ -241,6 +248,189 @@ def setup_fe_spaces(mesh: CartesianMesh, u_tags, j_tags, thin_walls: Sequence[di
         order = np.lexsort((fl, fc))
         sp.thin_wall_faces.append(dict(cell=fc[order].astype(np.int32), lface=fl[order].astype(np.int8),
                                        tau=float(tw["tau"]), cw=float(tw["cw"]), jw=float(tw.get("jw", 0.0))))
+    return sp
+
+
+def _first_appearance_rows(rows: np.ndarray) -> (np.ndarray, int):
+    """Ids 0..n-1 of the rows of ``rows`` (k, m) in first-appearance order."""
+    _, first, inv = np.unique(rows, axis=0, return_index=True, return_inverse=True)
+    inv = inv.reshape(-1)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    return rank[inv], first.size
+
+
+def _face_gauss_points(cell_xyz: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    """Physical positions (n, 4, 3) of the 2x2 face Gauss points (the RT face
+    dof points, k = g1 + 2 g2) of local faces ``faces`` (n,) of cells ``cell_xyz`` (n, 8, 3)."""
+    n = faces.size
+    ref = np.zeros((6, 4, 3))
+    for f in range(6):
+        d, t = FACE_AXIS[f], _tangential_axes(FACE_AXIS[f])
+        for k in range(4):
+            ref[f, k, d] = float(FACE_SIDE[f])
+            ref[f, k, t[0]] = G2[k % 2]
+            ref[f, k, t[1]] = G2[k // 2]
+    xi = ref[faces]
+    N, _ = _q1_shape(xi.reshape(-1, 3))
+    return np.einsum("nkv,nvi->nki", N.reshape(n, 4, 8), cell_xyz)
+
+
+def _setup_hex_spaces(mesh, u_tags, j_tags, thin_walls, style) -> "Spaces":
+    """``setup_fe_spaces`` on an unstructured hex mesh (``mesh_io.HexMesh``).
+
+    Topology: vertices are the mesh nodes used by cells (in node order),
+    edges and faces get ids in first-appearance order over cells and local
+    edges / faces.  Q2 nodes and their tags: a node is Dirichlet for a tag if
+    its geometric entity (the gmsh node, line or quad classification) is in
+    the tag, or it lies on the closure of a boundary quad of the tag.  RT face
+    dofs belong to the first cell that has the face; another cell sharing it
+    sees the same four Gauss points in its own face coordinates (matched by
+    physical position) and the normal of its local axis either way (sign
+    +1 when the face is on opposite local sides of the two cells, -1 when on
+    the same side)."""
+    from .mesh_io import EDGE_VERTS, FACE_VERTS
+    cells = mesh.cells
+    ncells = mesh.ncells
+    used, vid = np.unique(cells, return_inverse=True)
+    vid = vid.reshape(ncells, 8)
+    nv = used.size
+    ev = np.sort(vid[:, EDGE_VERTS], axis=2)                        # (C, 12, 2)
+    edge_id, n_edges = _first_appearance_ids(ev[..., 0].astype(np.int64) * nv + ev[..., 1])
+    fv = np.sort(vid[:, FACE_VERTS], axis=2)                        # (C, 6, 4)
+    fid_flat, n_faces = _first_appearance_rows(fv.reshape(-1, 4))
+    face_id = fid_flat.reshape(ncells, 6)
+    # local Q2 node -> (kind, local entity)
+    node_of_local = np.empty((ncells, 27), dtype=np.int64)
+    for a, (i, j, k) in enumerate(Q2_OFFSETS):
+        ones = [ax for ax, v in enumerate((i, j, k)) if v == 1]
+        b = [v // 2 for v in (i, j, k)]
+        if not ones:
+            node_of_local[:, a] = vid[:, b[0] + 2 * b[1] + 4 * b[2]]
+        elif len(ones) == 1:
+            e = ones[0]
+            le = {0: b[1] + 2 * b[2], 1: 4 + b[0] + 2 * b[2], 2: 8 + b[0] + 2 * b[1]}[e]
+            node_of_local[:, a] = nv + edge_id[:, le]
+        elif len(ones) == 2:
+            d = ({0, 1, 2} - set(ones)).pop()
+            lf = 2 * (2 - d) + (i, j, k)[d] // 2
+            node_of_local[:, a] = nv + n_edges + face_id[:, lf]
+        else:
+            node_of_local[:, a] = nv + n_edges + n_faces + np.arange(ncells)
+    n_nodes = nv + n_edges + n_faces + ncells
+    # boundary quads: (cell, local face, entity)
+    bq = mesh.boundary_quads()
+    face_edges = [[e for e in range(12) if set(EDGE_VERTS[e]) <= set(FACE_VERTS[f])] for f in range(6)]
+
+    def tagged_nodes(tags):
+        ents = mesh.entities(tags)
+        out = np.zeros(n_nodes, dtype=bool)
+        if not ents:
+            return out
+        ne = np.array([tuple(e) in ents for e in mesh.node_entity[used]])
+        out[:nv] = ne
+        for key, e in mesh.line_entity.items():
+            if e in ents:
+                a_, b_ = np.searchsorted(used, key)
+                hit = (ev[..., 0] == min(a_, b_)) & (ev[..., 1] == max(a_, b_))
+                out[nv + edge_id[hit]] = True
+        for c, f, e in bq:
+            if e in ents:
+                out[nv + n_edges + face_id[c, f]] = True
+                out[vid[c, FACE_VERTS[f]]] = True
+                out[nv + edge_id[c, face_edges[f]]] = True
+        return out
+
+    u_dir_node = tagged_nodes(u_tags) if u_tags else np.zeros(n_nodes, bool)
+    node_free = ~u_dir_node
+    u_node_dof = np.empty(n_nodes, dtype=np.int64)
+    fr = np.cumsum(node_free) - node_free
+    dr = np.cumsum(~node_free) - (~node_free)
+    u_node_dof[node_free] = 3 * fr[node_free]
+    u_node_dof[~node_free] = -(3 * dr[~node_free]) - 1
+    n_u_free = 3 * int(node_free.sum())
+    n_u_dir = 3 * int((~node_free).sum())
+    # RT: face owners, point permutations, normal signs
+    flat_first = np.full(n_faces, -1, dtype=np.int64)
+    order = np.arange(ncells * 6)
+    flat_first[fid_flat[::-1]] = order[::-1]                       # first (cell, face) of each face
+    owner_c, owner_f = flat_first // 6, flat_first % 6
+    allc = np.repeat(np.arange(ncells), 6)
+    allf = np.tile(np.arange(6), ncells)
+    xyz = mesh.cell_xyz()
+    P = _face_gauss_points(xyz[allc], allf)                         # (C*6, 4, 3)
+    Po = P[flat_first[fid_flat]]                                    # owner's points of every cell face
+    dist = np.linalg.norm(P[:, :, None, :] - Po[:, None, :, :], axis=3)   # (C*6, 4 own, 4 owner)
+    perm = np.argmin(dist, axis=2)
+    scale = np.linalg.norm(Po[:, 0] - Po[:, 3], axis=1)[:, None] + 1e-300
+    if not (np.take_along_axis(dist, perm[..., None], 2)[..., 0] <= 1e-8 * scale).all() or \
+            not (np.sort(perm, axis=1) == np.arange(4)).all():
+        raise ValueError("a shared face's Gauss points do not match between its cells (non-conforming mesh?)")
+    side = FACE_SIDE[allf]
+    is_owner = flat_first[fid_flat] == np.arange(ncells * 6)
+    sign = np.where(is_owner | (side != FACE_SIDE[owner_f[fid_flat]]), 1, -1).reshape(ncells, 6)
+    perm = perm.reshape(ncells, 6, 4)
+    def tagged_faces(tags):
+        ents = mesh.entities(tags)
+        out = np.zeros(n_faces, dtype=bool)
+        for c, f, e in bq:
+            if e in ents:
+                out[face_id[c, f]] = True
+        return out
+
+    # Dirichlet faces of j
+    face_dir = tagged_faces(j_tags) if j_tags else np.zeros(n_faces, dtype=bool)
+    face_free = ~face_dir
+    ffr = np.cumsum(face_free) - face_free
+    fdr = np.cumsum(~face_free) - (~face_free)
+    face_dof = np.empty(n_faces, dtype=np.int64)
+    face_dof[face_free] = 4 * ffr[face_free]
+    face_dof[~face_free] = -(4 * fdr[~face_free]) - 1
+    n_j_face_free = 4 * int(face_free.sum())
+    n_j_free = n_j_face_free + 12 * ncells
+    n_j_dir = 4 * int((~face_free).sum())
+    n_p, n_phi = 4 * ncells, 8 * ncells
+    if style == "badia2024":
+        off_u, off_j = 0, n_u_free
+        off_p = off_j + n_j_free
+        off_phi = off_p + n_p
+    elif style == "consecutive":
+        off_u, off_p = 0, n_u_free
+        off_j = off_p + n_p
+        off_phi = off_j + n_j_free
+    else:
+        raise ValueError(f"unknown multifield style {style!r}")
+    cd = np.empty((ncells, NDOF), dtype=np.int64)
+    nd = u_node_dof[node_of_local]
+    for c in range(3):
+        free = nd >= 0
+        cd[:, c * 27:(c + 1) * 27] = np.where(free, off_u + nd + c, -((-nd - 1) + c) - 1)
+    cidx = np.arange(ncells, dtype=np.int64)
+    cd[:, OFF_P:OFF_P + 4] = off_p + 4 * cidx[:, None] + np.arange(4)[None]
+    fd = face_dof[face_id]
+    for f in range(6):
+        v = fd[:, f]
+        for k in range(4):
+            kk = perm[:, f, k]
+            cd[:, OFF_J + 4 * f + k] = np.where(v >= 0, off_j + v + kk, -(n_u_dir + (-v - 1) + kk) - 1)
+    cd[:, OFF_J + 24:OFF_J + 36] = off_j + n_j_face_free + 12 * cidx[:, None] + np.arange(12)[None]
+    cd[:, OFF_PHI:OFF_PHI + 8] = off_phi + 8 * cidx[:, None] + np.arange(8)[None]
+    assert cd.max() < 2**31
+    sp = Spaces(mesh=mesh, nfree={"u": n_u_free, "p": n_p, "j": n_j_free, "phi": n_phi},
+                ndir={"u": n_u_dir, "j": n_j_dir}, cell_dofs=cd.astype(np.int32), rt_sign=sign.astype(np.int8),
+                dir_vals=np.zeros(n_u_dir + n_j_dir), u_node_rank=node_of_local, u_node_free=node_free,
+                u_node_dof=u_node_dof, face_ids=face_id, face_dirichlet=face_dir, face_dof=face_dof,
+                cell_verts=vid.astype(np.int32), style=style,
+                bc_spec=dict(u_tags=u_tags, j_tags=j_tags, thin_walls=list(thin_walls)),
+                node_tagger=tagged_nodes, face_tagger=tagged_faces)
+    for tw in thin_walls:
+        ents = mesh.entities(tw["domain"])
+        sel = sorted((c, f) for c, f, e in bq if e in ents)
+        fc = np.array([c for c, _ in sel], dtype=np.int32)
+        fl = np.array([f for _, f in sel], dtype=np.int8)
+        sp.thin_wall_faces.append(dict(cell=fc, lface=fl, tau=float(tw["tau"]), cw=float(tw["cw"]),
+                                       jw=float(tw.get("jw", 0.0))))
     return sp
 
 
@@ -343,6 +533,38 @@ def interpolate_phi(sp: Spaces, fun: Callable):
     """dQ1 values at the 8 cell vertices."""
     xyz = sp.mesh.cell_xyz()
     return np.asarray(fun(xyz.reshape(-1, 3)), dtype=np.float64).reshape(-1)
+
+
+def dirichlet_values_by_tag(sp: Spaces, field: str, tags, fns) -> np.ndarray:
+    """Dirichlet values of ``field`` ("u" or "j") with one function per tag
+    (``TrialFESpace(V, [f1, f2, ...])`` over ``dirichlet_tags = [t1, t2, ...]``,
+    src/fespaces.jl:116-124): a Dirichlet dof takes the function of the first
+    tag of the list that holds it."""
+    if isinstance(tags, str):
+        tags = [tags]
+    if len(tags) != len(fns):
+        raise ValueError(f"{len(fns)} values for {len(tags)} Dirichlet tags of {field}")
+    interp = interpolate_u if field == "u" else interpolate_j
+    vals = [interp(sp, f)[1] for f in fns]
+    out = np.zeros(sp.ndir[field])
+    done = np.zeros(sp.ndir[field], dtype=bool)
+    if field == "u":
+        owner = sp.u_node_dof
+        mask_of = sp.node_tagger
+        width = 3
+    else:
+        owner = sp.face_dof
+        mask_of = sp.face_tagger
+        width = 4
+    dirs = np.nonzero(owner < 0)[0]
+    for t, v in zip(tags, vals):
+        m = mask_of(t)[dirs]
+        first = -owner[dirs[m]] - 1
+        idx = (first[:, None] + np.arange(width)[None]).ravel()
+        take = ~done[idx]
+        out[idx[take]] = v[idx[take]]
+        done[idx] = True
+    return out
 
 
 def interpolate(sp: Spaces, u=None, p=None, j=None, phi=None):

@@ -67,12 +67,17 @@ def setup_fe_spaces(params) -> fespaces.Spaces:
     thin = [dict(domain=tw["domain"], cw=tw["cw"], tau=tw["τ"], jw=tw["jw"]) for tw in (bcs["thin_wall"] or [])]
     sp = fespaces.setup_fe_spaces(mesh, bcs["u"]["tags"], bcs["j"]["tags"], thin, style="badia2024")
     # Dirichlet values (TrialFESpace(V, u_bc), src/fespaces.jl:116-124)
-    ufn, jfn = _values_fn(bcs["u"].get("values")), _values_fn(bcs["j"].get("values"))
     dv = np.zeros(sp.ndir["u"] + sp.ndir["j"])
-    if ufn is not None:
-        dv[:sp.ndir["u"]] = fespaces.interpolate_u(sp, ufn)[1]
-    if jfn is not None:
-        dv[sp.ndir["u"]:] = fespaces.interpolate_j(sp, jfn)[1]
+    for f, sl in (("u", slice(0, sp.ndir["u"])), ("j", slice(sp.ndir["u"], None))):
+        vals = bcs[f].get("values")
+        if vals is None:
+            continue
+        if isinstance(vals, (list, tuple)) and len(vals) and (callable(vals[0]) or np.ndim(vals[0]) == 1):
+            # one value per Dirichlet tag
+            dv[sl] = fespaces.dirichlet_values_by_tag(sp, f, bcs[f]["tags"], [_values_fn(v) for v in vals])
+        else:
+            fn = _values_fn(vals)
+            dv[sl] = (fespaces.interpolate_u if f == "u" else fespaces.interpolate_j)(sp, fn)[1]
     sp.dir_vals = dv
     return sp
 

@@ -136,8 +136,15 @@ def params_fluid(params):
     fluid = _check_and_add(params["fluid"], mandatory, optional, params, "[:fluid]")
     if fluid["convection"] not in ("none", "newton", "picard"):
         raise ParamsError("fluid[:convection] must be one of none, newton, picard")
-    if fluid["domain"] is not None:
-        raise NotImplementedError("fluid sub-domains (solid regions) are outside the hot path")
+    dom = fluid["domain"]
+    if dom is not None:
+        # a tag is accepted when it covers every cell (the fluid is the whole mesh)
+        model = params.get("model")
+        groups = getattr(model, "groups", None)
+        cells_in = groups is not None and dom in groups and all(
+            tuple(e) in groups[dom] for e in getattr(model, "cell_entity", []))
+        if not cells_in:
+            raise NotImplementedError("fluid sub-domains (solid regions) are outside the hot path")
     return fluid
 
 

@@ -131,3 +131,21 @@ def general_hunt_error(sp, x_free, l, d_b, grad_pz, Ha, nsums, nq=4):
     e = np.sum(dV * np.einsum("cpk,cpk->cp", ua - u, ua - u))
     n = np.sum(dV * np.einsum("cpk,cpk->cp", ua, ua))
     return float(np.sqrt(e / n))
+
+
+def write_vtk(path, xh):
+    """``writevtk`` of the solution fields (src/Applications/hunt.jl:263-273,
+    expansion.jl:246-258) as a VTK unstructured grid: u, j, p, phi and div u
+    at the 8 vertices of every cell (each cell keeps its own points, so the
+    discontinuous fields are written as they are)."""
+    from . import mesh_io
+    sp, x = xh.spaces, xh.free_values
+    corners = np.array([[v & 1, (v >> 1) & 1, (v >> 2) & 1] for v in range(8)], dtype=np.float64)
+    X, u, G, j, _ = eval_fields(sp, x, corners)
+    loc = local_values(sp, x)
+    pb = np.stack([1 - corners.sum(1), corners[:, 0], corners[:, 1], corners[:, 2]], 1)   # (8, 4) dP1 basis
+    p = loc[:, 81:85] @ pb.T
+    phi = loc[:, 121:129]
+    divu = np.trace(G, axis1=2, axis2=3)
+    return mesh_io.write_vtu(path, sp.mesh.cell_xyz(), point_data={"uh": u, "jh": j, "ph": p, "phi": phi,
+                                                                     "div_uh": divu})

@@ -153,6 +153,26 @@ def adjoint_case(name, zeta=None):
     return adj.inlet_duct(kw["nc"], kw["Ha"], kw["Re"], kw["cw"], zeta=zeta, jitter=jitter)
 
 
+EXPANSION_MSH = os.path.join(FIXTURE_DIR, "Expansion_710.msh")
+# the expansion application (src/Applications/expansion.jl) on the reference's
+# 240-cell gmsh mesh: insulating walls, and thin conducting walls
+EXPANSION_CASES = {
+    "expansion_710": dict(Ha=1.0, N=1.0, solid_coupling="none"),
+    "expansion_710_thin_wall": dict(Ha=10.0, N=10.0, cw=0.028, solid_coupling="thin_wall"),
+}
+
+
+def expansion_case(name):
+    """(params, spaces, physics) of an expansion case as main() builds them."""
+    import importlib
+    from gridapmhd_amd import applications, parameters
+    m = importlib.import_module("gridapmhd_amd.main")
+    params = applications.expansion_params(mesh=EXPANSION_MSH, **EXPANSION_CASES[name])
+    full = parameters.add_default_params(params)
+    sp = m.setup_fe_spaces(full)
+    return params, sp, m.fluid_physics(full["fluid"])
+
+
 def load_fixture(name):
     with np.load(os.path.join(FIXTURE_DIR, f"oracle_{name}.npz"), allow_pickle=False) as z:
         return {k: z[k] for k in z.files}

@@ -4,6 +4,8 @@ and store the results as fixtures (tests/golden/oracle_<name>.npz).
 * c2_params_8x8x8: the thin-wall duct at the 64 x 64 x 8 benchmark's physics
   (Ha = 100, Re = 10, cw = 0.01, Newton convection) on 8 x 8 x 8 cells:
   Newton with a direct solve per step (the plain method, zeta = 0).
+* the expansion cases (tests/cases.py EXPANSION_CASES): the expansion
+  application on the reference's 240-cell gmsh mesh, Newton + direct solve.
 * the developing-duct adjoint cases (tests/cases.py ADJOINT_CASES): the
   state (Newton + direct solve), the adjoint lambda = A(x)^-T dJ/dx on the
   same factorisation of A(x) with iterative refinement, and dJ/d(Ha, cw) = -lambda . dR/d(Ha, cw) with
@@ -98,13 +100,29 @@ def make_adjoint(name):
     return dict(x=x, lam=lam, meta=np.array(json.dumps(meta)))
 
 
+def make_expansion(name):
+    _, sp, phys = cases.expansion_case(name)
+    orc = oracle.Oracle(cases.problem_dict(sp, phys))
+    # the outlet leaves u free (no pressure mode); j.n = 0 on the whole
+    # boundary leaves the constant potential free unless thin walls carry current
+    pin_phi = cases.EXPANSION_CASES[name]["solid_coupling"] == "none"
+    x, hist = newton(orc, np.zeros(sp.ndofs), pins=oracle.null_pins(sp, pin_p=False, pin_phi=pin_phi))
+    meta = dict(case=name, **cases.EXPANSION_CASES[name], newton_residuals=hist, ndofs=sp.ndofs)
+    return dict(x=x, meta=np.array(json.dumps(meta)))
+
+
 def main(names):
     oracle.build()
-    todo = names or ["c2_params_8x8x8"] + list(cases.ADJOINT_CASES)
+    todo = names or ["c2_params_8x8x8"] + list(cases.ADJOINT_CASES) + list(cases.EXPANSION_CASES)
     for name in todo:
         t = time.perf_counter()
         print(f"[{name}]", flush=True)
-        data = make_c2_params() if name == "c2_params_8x8x8" else make_adjoint(name)
+        if name == "c2_params_8x8x8":
+            data = make_c2_params()
+        elif name in cases.EXPANSION_CASES:
+            data = make_expansion(name)
+        else:
+            data = make_adjoint(name)
         np.savez_compressed(os.path.join(HERE, f"oracle_{name}.npz"), **data)
         print(f"[{name}] done in {time.perf_counter() - t:.0f} s", flush=True)
 

@@ -55,3 +55,15 @@ def test_adjoint_fixture_solves_the_oracle(name, orc_mod):
     assert abs(float(g @ x) / meta["J"] - 1) < 1e-13
     assert abs(-(2 * Ha / Re) * float(lam @ dRg) / meta["dJ_dHa"] - 1) < 1e-12
     assert abs(-float(lam @ dRc) / meta["dJ_dcw"] - 1) < 1e-12
+
+
+@pytest.mark.parametrize("name", list(cases.EXPANSION_CASES))
+def test_expansion_fixture_solves_the_oracle(name, orc_mod):
+    fx = cases.load_fixture(name)
+    _, sp, phys = cases.expansion_case(name)
+    orc = orc_mod.Oracle(cases.problem_dict(sp, phys))
+    x = fx["x"]
+    r0 = np.linalg.norm(orc.residual(np.zeros(sp.ndofs)))
+    rel = np.linalg.norm(orc.residual(x)) / r0
+    assert rel < 1e-12, rel
+    assert np.linalg.norm(orc.residual(x * (1 + 1e-9))) / r0 > 10 * rel

@@ -1,0 +1,203 @@
+"""Unstructured hex meshes (gmsh 4.1 reader, spaces on them, VTK output).
+
+* The reference's Expansion_710 mesh (meshes/Expansion_710.msh, kept as data
+  in tests/golden/): cell count, tags, the measures of the fluid volume and
+  of the inlet / outlet / wall surfaces (the expansion geometry of
+  src/Applications/expansion.jl:272-275: a 0.5 x 2 inlet channel and a 2 x 2
+  outlet channel, both 8 long).
+* Orientation and numbering: a Cartesian box written as an unstructured mesh
+  with shuffled nodes and cells and every cell's local vertices rotated by a
+  random cube rotation must give the same discrete problem as the structured
+  spaces: fields of the spaces interpolate to the same integrals, and the
+  oracle's Newton solutions have the same integrals.  A wrong RT face
+  permutation or normal sign breaks both."""
+import itertools
+import os
+import xml.etree.ElementTree as ET
+
+import numpy as np
+import pytest
+
+import cases
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MSH = os.path.join(HERE, "golden", "Expansion_710.msh")
+
+
+def _rotations():
+    out = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            R = np.zeros((3, 3), dtype=np.int64)
+            for i, p in enumerate(perm):
+                R[i, p] = signs[i]
+            if round(np.linalg.det(R)) == 1:
+                out.append(R)
+    assert len(out) == 24
+    return out
+
+
+def _scramble(hm, seed):
+    """Same mesh: shuffled nodes and cells, every cell's vertices rotated."""
+    from gridapmhd_amd import mesh_io
+    rng = np.random.default_rng(seed)
+    corners = np.array([[(v >> a) & 1 for a in range(3)] for v in range(8)]) * 2 - 1
+    rots = _rotations()
+    cells = hm.cells.copy()
+    for c in range(cells.shape[0]):
+        R = rots[rng.integers(24)]
+        old = (R @ corners.T).T
+        idx = [int(np.nonzero((corners == o).all(1))[0][0]) for o in old]
+        cells[c] = hm.cells[c, idx]
+    nn = hm.node_xyz.shape[0]
+    p = rng.permutation(nn)                     # old node -> new node
+    inv = np.argsort(p)
+    co = rng.permutation(cells.shape[0])
+    remap = lambda key: tuple(sorted(int(p[k]) for k in key))
+    return mesh_io.HexMesh(node_xyz=hm.node_xyz[inv], cells=p[cells][co], node_entity=hm.node_entity[inv],
+                           cell_entity=hm.cell_entity[co],
+                           quad_entity={remap(k): v for k, v in hm.quad_entity.items()},
+                           line_entity={remap(k): v for k, v in hm.line_entity.items()},
+                           groups={k: set(v) for k, v in hm.groups.items()})
+
+
+def _planar_quad_area(P):
+    """Area of a planar convex quad given its 4 corners in any order."""
+    c = P.mean(0)
+    n = np.cross(P[1] - P[0], P[2] - P[0])
+    if np.linalg.norm(n) < 1e-14:
+        n = np.cross(P[1] - P[0], P[3] - P[0])
+    n /= np.linalg.norm(n)
+    e1 = (P[0] - c) / np.linalg.norm(P[0] - c)
+    e2 = np.cross(n, e1)
+    Q = P[np.argsort(np.arctan2((P - c) @ e2, (P - c) @ e1))]
+    return 0.5 * np.linalg.norm(np.cross(Q[2] - Q[0], Q[3] - Q[1]))
+
+
+def _fields():
+    u = lambda X: np.stack([X[:, 0] ** 2 + 0.3 * X[:, 1] * X[:, 2] - 0.2, X[:, 0] * X[:, 1] + 0.7 * X[:, 2] ** 2,
+                            X[:, 1] ** 2 - X[:, 0] * X[:, 2]], 1)
+    j = lambda X: np.stack([0.4 * X[:, 0] ** 2 * X[:, 1] + X[:, 2] - 0.3 * X[:, 0] * X[:, 2] + 1.0,
+                            X[:, 1] ** 2 * X[:, 0] - 0.2 * X[:, 1] + 0.5 * X[:, 2] * X[:, 0],
+                            X[:, 2] ** 2 + 0.6 * X[:, 0] * X[:, 1] - 0.1 * X[:, 2] * X[:, 1]], 1)
+    p = lambda X: 0.3 + X[:, 0] - 2.0 * X[:, 1] + 0.5 * X[:, 2]
+    phi = lambda X: 1.0 - X[:, 0] + 0.25 * X[:, 2]
+    return u, j, p, phi
+
+
+def _box():
+    import gridapmhd_amd as g
+    cm = g.meshers.cartesian_model((-1.0, 1.0, -1.0, 1.0, 0.0, 0.6), (3, 3, 2), (False, False, False),
+                                   map=g.meshers.tanh_map(1.5))
+    g.meshers.hunt_add_tags(cm)
+    cm.add_tag_from_tags("ends", [21, 22])
+    return cm
+
+
+def test_expansion_mesh_reads():
+    from gridapmhd_amd import mesh_io
+    m = mesh_io.read_gmsh(MSH)
+    assert m.ncells == 240 and m.node_xyz.shape == (402, 3)
+    assert sorted(m.groups) == ["fluid", "inlet", "outlet", "wall"]
+    X = m.cell_xyz()
+    # volumes by the 8-point rule of the trilinear map (exact for trilinear cells)
+    from gridapmhd_amd import fespaces
+    g = np.array([0.5 - 0.5 / np.sqrt(3), 0.5 + 0.5 / np.sqrt(3)])
+    xi = np.array([[a, b, c] for c in g for b in g for a in g])
+    _, J = fespaces.map_points(X, xi)
+    vol = np.abs(np.linalg.det(J)).sum() / 8.0
+    assert abs(vol - 40.0) < 1e-9, vol
+    assert (np.linalg.det(J) > 0).all()
+    # every boundary face (a face of one cell) is a tagged quad
+    from gridapmhd_amd.mesh_io import FACE_VERTS
+    fv = np.sort(m.cells[:, FACE_VERTS], axis=2).reshape(-1, 4)
+    keys, counts = np.unique(fv, axis=0, return_counts=True)
+    bnd = {tuple(k) for k in keys[counts == 1]}
+    assert bnd == set(m.quad_entity)
+    area = {}
+    for key, e in m.quad_entity.items():
+        ar = _planar_quad_area(m.node_xyz[list(key)])
+        for name in ("inlet", "outlet", "wall"):
+            if e in m.groups[name]:
+                area[name] = area.get(name, 0.0) + ar
+    assert abs(area["inlet"] - 1.0) < 1e-9 and abs(area["outlet"] - 4.0) < 1e-9, area
+    # walls: the two channels' side walls plus the step at x = 0
+    assert abs(area["wall"] - (2 * 8 * (0.5 + 2) + 2 * 8 * (2 + 2) + (4 - 1))) < 1e-9, area
+
+
+def test_unstructured_spaces_match_structured(orc_mod):
+    import gridapmhd_amd as g
+    from gridapmhd_amd import fespaces, mesh_io
+    cm = _box()
+    hm = _scramble(mesh_io.hexmesh_from_cartesian(cm), 11)
+    tw = [dict(domain="insulating", cw=0.05, tau=100.0, jw=0.0)]
+    sp_s = fespaces.setup_fe_spaces(cm, "noslip", "ends", tw, style="badia2024")
+    sp_u = fespaces.setup_fe_spaces(hm, "noslip", "ends", tw, style="badia2024")
+    assert sp_s.nfree == sp_u.nfree and sp_s.ndir == sp_u.ndir
+    assert (sp_u.rt_sign == -1).any()          # the rotations do flip local normals
+    assert sum(t["cell"].size for t in sp_s.thin_wall_faces) == sum(t["cell"].size for t in sp_u.thin_wall_faces)
+    u, j, p, phi = _fields()
+    phys = cases.cfd_physics(10.0, 5.0)
+    norms = []
+    for sp in (sp_s, sp_u):
+        x, dv = fespaces.interpolate(sp, u=u, j=j, p=p, phi=phi)
+        orc = orc_mod.Oracle(cases.problem_dict(sp, phys, dir_vals=dv))
+        norms.append(orc.cell_norms(x).sum(0))
+    assert np.allclose(norms[0], norms[1], rtol=1e-12, atol=1e-13), norms
+    # the discrete problems: Newton solutions with interpolated Dirichlet data
+    sols = []
+    for sp in (sp_s, sp_u):
+        # a divergence-free wall velocity (its flux through the closed walls vanishes)
+        wall = lambda X: np.stack([0.5 * X[:, 1], -0.5 * X[:, 0], 0.0 * X[:, 0]], 1)
+        _, dv = fespaces.interpolate(sp, u=wall, j=lambda X: 0.0 * X)
+        orc = orc_mod.Oracle(cases.problem_dict(sp, phys, dir_vals=dv))
+        # u is free on the two z ends (natural condition): no pressure null mode
+        x = orc_mod.newton_direct(orc, np.zeros(sp.ndofs), rtol=1e-13, atol=1e-15)
+        nrm = orc.cell_norms(x)
+        sols.append(np.concatenate([nrm[:, :4].sum(0), [np.abs(orc.residual(x)).max()]]))
+    assert sols[0][4] < 1e-12 and sols[1][4] < 1e-12
+    assert np.allclose(sols[0][:4], sols[1][:4], rtol=1e-10), sols
+
+
+def test_gmsh_round_trip_and_vtk(tmp_path):
+    from gridapmhd_amd import fespaces, mesh_io
+    hm = _scramble(mesh_io.hexmesh_from_cartesian(_box()), 5)
+    path = str(tmp_path / "box.msh")
+    mesh_io.write_gmsh(path, hm)
+    back = mesh_io.read_gmsh(path)
+    assert np.array_equal(back.cells, hm.cells) and np.allclose(back.node_xyz, hm.node_xyz, rtol=0, atol=0)
+    assert back.quad_entity == hm.quad_entity and back.line_entity == hm.line_entity
+    assert {k: v for k, v in back.groups.items() if not k.startswith("physical")} == hm.groups
+    a = fespaces.setup_fe_spaces(hm, "noslip", "ends", style="badia2024")
+    b = fespaces.setup_fe_spaces(back, "noslip", "ends", style="badia2024")
+    assert np.array_equal(a.cell_dofs, b.cell_dofs) and np.array_equal(a.rt_sign, b.rt_sign)
+    X = hm.cell_xyz()
+    vtu = mesh_io.write_vtu(str(tmp_path / "box"), X, point_data={"u": X * 2.0, "s": X[..., 0]},
+                            cell_data={"c": np.arange(hm.ncells, dtype=float)})
+    root = ET.parse(vtu).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    assert int(piece.get("NumberOfCells")) == hm.ncells and int(piece.get("NumberOfPoints")) == 8 * hm.ncells
+    pts = np.array(piece.find("Points/DataArray").text.split(), float).reshape(-1, 3)
+    # VTK hexahedron order: the 4 bottom vertices counter-clockwise, then the top
+    assert np.allclose(pts[:8], X[0][[0, 1, 3, 2, 4, 5, 7, 6]])
+
+
+def test_expansion_dirichlet_values_follow_tag_order():
+    """u = inlet profile on the inlet (the first tag of the list, so also on
+    its rim), 0 on the remaining wall nodes; j.n = 0 on every boundary face
+    (src/Applications/expansion.jl:150-168)."""
+    from gridapmhd_amd import applications, fespaces
+    params, sp, _ = cases.expansion_case("expansion_710")
+    u_in = applications.u_inlet("parabolic", 1.0, 4.0, 0.2)
+    inlet = sp.node_tagger(["inlet"])
+    wall = sp.node_tagger(["wall"])
+    dirn = np.nonzero(sp.u_node_dof < 0)[0]
+    assert set(dirn) == set(np.nonzero(inlet | wall)[0])
+    all_in = fespaces.interpolate_u(sp, u_in)[1]
+    first = -sp.u_node_dof[dirn] - 1
+    got = sp.dir_vals[first[:, None] + np.arange(3)]
+    want_in = all_in[first[:, None] + np.arange(3)]
+    on_in = inlet[dirn]
+    assert np.array_equal(got[on_in], want_in[on_in]) and np.abs(want_in[on_in]).max() > 5.0
+    assert not got[~on_in].any()
+    assert sp.ndir["j"] == 4 * len(sp.mesh.quad_entity) and not sp.dir_vals[sp.ndir["u"]:].any()
