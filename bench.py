@@ -247,7 +247,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--asm-reps", type=int, default=10)
-    ap.add_argument("--mixed", type=int, default=1, choices=[1, 2], help="mixed mode: 1 FP32 / 2 BF16 patch blocks")
+    ap.add_argument("--mixed", type=int, default=1, choices=[1, 2, 3],
+                    help="mixed mode: 1 FP32 / 2 BF16 / 3 FP16 (scaled per patch) patch blocks")
     ap.add_argument("--restart", type=int, default=200)
     ap.add_argument("--inner", type=int, default=3, help="flexible-GMRES steps of the inexact [u,j] solve")
     ap.add_argument("--levels", default="auto", help="levels of the [u,j] V-cycle (1 = Schwarz only; auto: 5 at C2)")
@@ -450,7 +451,7 @@ def main():
         ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh)
         t_sm[prec] = time_events(lambda: ctx.smoother_apply_d(rr.data_ptr(), yy.data_ptr(), sh), 10, stream)
     npatch, pdim, pent, npiv = ctx.smoother_info(pivoted=True)
-    esize = {0: 8, 1: 4, 2: 2}
+    esize = {0: 8, 1: 4, 2: 2, 3: 2}
     sm_bytes = {p: pent * esize[p] for p in t_sm}
     log(f"smoother apply FP64 {t_sm[0] * 1e3:.3f} ms ({sm_bytes[0] / t_sm[0] / 1e9:.0f} GB/s), "
         f"mixed {t_sm[args.mixed] * 1e3:.3f} ms ({sm_bytes[args.mixed] / t_sm[args.mixed] / 1e9:.0f} GB/s), "

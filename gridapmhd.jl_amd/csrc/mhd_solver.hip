@@ -14,14 +14,16 @@
 // with CG-Jacobi).  A11^-1 is approximated by `inner_iters` steps of GMRES on
 // A11, right-preconditioned by additive Schwarz over vertex-star patches (the
 // patch of a vertex holds every u / j dof whose support lies in the cells
-// around it); the dense patch inverses come from batched LU.
+// around it); the dense patch inverses come from a batched Gauss-Jordan kernel.
 //
-// Precision: precision 0 runs everything in FP64.  Precisions 1 and 2 keep
+// Precision: precision 0 runs everything in FP64.  Precisions 1 to 3 keep
 // FP64 for the residual assembly, the outer FGMRES (the refinement residual,
 // its classical Gram-Schmidt with reorthogonalisation, the Newton update) and
 // the p / phi Schur solves, and run the inner A11 GMRES in FP32 with an FP32
-// copy of A11; the patch inverses are stored in FP32 (1) or BF16 (2).
+// copy of A11; the patch inverses are stored in FP32 (1), BF16 (2) or FP16
+// scaled by a power of two per patch (3).  Patch products sum in FP64.
 #include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -45,6 +47,8 @@ template <class T>
 __device__ inline double todbl(T v) { return (double)v; }
 template <>
 __device__ inline double todbl<__hip_bfloat16>(__hip_bfloat16 v) { return (double)__bfloat162float(v); }
+template <>
+__device__ inline double todbl<__half>(__half v) { return (double)__half2float(v); }
 
 // ---------------------------------------------------------------- vectors
 template <class T>
@@ -378,6 +382,8 @@ template <class T>
 __device__ inline T from_dbl(double v) { return (T)v; }
 template <>
 __device__ inline __hip_bfloat16 from_dbl<__hip_bfloat16>(double v) { return __float2bfloat16((float)v); }
+template <>
+__device__ inline __half from_dbl<__half>(double v) { return __float2half((float)v); }
 
 constexpr int GJ_T = 512;              // threads per patch
 constexpr int GJ_TR = 16, GJ_TC = 32;  // thread grid: rows x columns
@@ -392,6 +398,7 @@ struct GJShared {
   int16_t pk[GJ_NMAX];
   int p, bad;
   double piv;
+  double wmax[GJ_T / 64];
 };
 
 template <class OutT>
@@ -412,19 +419,51 @@ __device__ __forceinline__ void gj_load(double (&S)[GJ_RI][GJ_RJ], GJShared &L, 
   if (t == 0) L.bad = 0;
 }
 
+// FP16 inverses are stored scaled by a power of two per patch, so that the
+// largest |entry| lies in [2^14, 2^15) (FP16 reaches 65504): pscale[pch] is the
+// factor the product is multiplied by again.  Other types store unscaled.
+template <class OutT>
+__device__ __forceinline__ double gj_scale(const double (&S)[GJ_RI][GJ_RJ], GJShared &L, int PS, int t, int tr,
+                                           int tc, float *__restrict__ pscale, int64_t pch) {
+  if constexpr (!std::is_same<OutT, __half>::value) {
+    return 1.0;
+  } else {
+    double m = 0.0;
+#pragma unroll
+    for (int i = 0; i < GJ_RI; ++i)
+#pragma unroll
+      for (int j = 0; j < GJ_RJ; ++j) {
+        const int r = tr + GJ_TR * i, c = tc + GJ_TC * j;
+        if (r < PS && c < PS) m = fmax(m, fabs(S[i][j]));
+      }
+    if (PS > GJ_NL) {
+      if (t < PS) m = fmax(m, fabs(L.R224[t]));
+      if (t < GJ_NL) m = fmax(m, fabs(L.C224[t]));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((t & 63) == 0) L.wmax[t >> 6] = m;
+    __syncthreads();
+    m = L.wmax[0];
+    for (int w = 1; w < GJ_T / 64; ++w) m = fmax(m, L.wmax[w]);
+    const int e = m > 0.0 ? ilogb(m) : 0;
+    if (t == 0) pscale[pch] = ldexpf(1.0f, e - 14);
+    return ldexp(1.0, 14 - e);
+  }
+}
+
 template <class OutT>
 __device__ __forceinline__ void gj_store(const double (&S)[GJ_RI][GJ_RJ], const GJShared &L, OutT *__restrict__ Op,
-                                         int PS, int t, int tr, int tc) {
+                                         int PS, int t, int tr, int tc, double sc) {
 #pragma unroll
   for (int i = 0; i < GJ_RI; ++i)
 #pragma unroll
     for (int j = 0; j < GJ_RJ; ++j) {
       const int r = tr + GJ_TR * i, c = tc + GJ_TC * j;
-      if (r < PS && c < PS) Op[(int64_t)c * PS + r] = from_dbl<OutT>(S[i][j]);
+      if (r < PS && c < PS) Op[(int64_t)c * PS + r] = from_dbl<OutT>(S[i][j] * sc);
     }
   if (PS > GJ_NL) {
-    if (t < PS) Op[(int64_t)t * PS + GJ_NL] = from_dbl<OutT>(L.R224[t]);
-    if (t < GJ_NL) Op[(int64_t)GJ_NL * PS + t] = from_dbl<OutT>(L.C224[t]);
+    if (t < PS) Op[(int64_t)t * PS + GJ_NL] = from_dbl<OutT>(L.R224[t] * sc);
+    if (t < GJ_NL) Op[(int64_t)GJ_NL * PS + t] = from_dbl<OutT>(L.C224[t] * sc);
   }
 }
 
@@ -518,7 +557,7 @@ __device__ __forceinline__ void gjd_step(double (&S)[GJ_RI][GJ_RJ], GJShared &L,
 template <class OutT>
 __global__ __launch_bounds__(GJ_T) void k_patch_gjd(int PS, double tau, const double *__restrict__ A,
                                                      OutT *__restrict__ out, int16_t *__restrict__ perm,
-                                                     int32_t *__restrict__ status) {
+                                                     int32_t *__restrict__ status, float *__restrict__ pscale) {
   __shared__ GJShared L;
   const int t = threadIdx.x, tr = t % GJ_TR, tc = t / GJ_TR;
   const int64_t pch = blockIdx.x;
@@ -577,7 +616,8 @@ __global__ __launch_bounds__(GJ_T) void k_patch_gjd(int PS, double tau, const do
     if (t < GJ_NL) L.C224[t] = -L.colbuf[t] * L.rowbuf[GJ_NL];
     __syncthreads();
   }
-  gj_store<OutT>(S, L, out + pch * PS * PS, PS, t, tr, tc);
+  const double sc = gj_scale<OutT>(S, L, PS, t, tr, tc, pscale, pch);
+  gj_store<OutT>(S, L, out + pch * PS * PS, PS, t, tr, tc, sc);
   int16_t *pp = perm + pch * 2 * PS;
   if (t < PS) {
     pp[t] = (int16_t)t;
@@ -703,7 +743,8 @@ __device__ __forceinline__ void gj_step(double (&S)[GJ_RI][GJ_RJ], GJShared &L, 
 template <class OutT>
 __global__ __launch_bounds__(GJ_T) void k_patch_gj(int PS, const int32_t *__restrict__ list,
                                                     const double *__restrict__ A, OutT *__restrict__ out,
-                                                    int16_t *__restrict__ perm, int32_t *__restrict__ status) {
+                                                    int16_t *__restrict__ perm, int32_t *__restrict__ status,
+                                                    float *__restrict__ pscale) {
   __shared__ GJShared L;
   const int t = threadIdx.x, tr = t % GJ_TR, tc = t / GJ_TR;
   const int64_t pch = list[blockIdx.x];
@@ -721,7 +762,8 @@ __global__ __launch_bounds__(GJ_T) void k_patch_gj(int PS, const int32_t *__rest
     }
   }
   gj_step(S, L, t, tr, tc, GJ_RJ, 0);
-  gj_store<OutT>(S, L, out + pch * PS * PS, PS, t, tr, tc);
+  const double sc = gj_scale<OutT>(S, L, PS, t, tr, tc, pscale, pch);
+  gj_store<OutT>(S, L, out + pch * PS * PS, PS, t, tr, tc, sc);
   int16_t *pp = perm + pch * 2 * PS;
   if (t < PS) {
     pp[t] = L.pk[t];
@@ -737,7 +779,7 @@ template <class MP, class T>
 __global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const int32_t *__restrict__ psize,
                                                      const int32_t *__restrict__ pidx, const int16_t *__restrict__ perm,
                                                      const MP *__restrict__ Ainv, const T *__restrict__ r,
-                                                     double *__restrict__ part) {
+                                                     double *__restrict__ part, const float *__restrict__ pscale) {
   __shared__ double rl[256];
   const int pch = blockIdx.x;
   const int sz = psize[pch];
@@ -757,7 +799,8 @@ __global__ __launch_bounds__(256) void k_patch_apply(int npatch, int PS, const i
       a3 += todbl(Ap[(int64_t)(b + 3) * PS]) * rl[b + 3];
     }
     for (; b < sz; ++b) a0 += todbl(Ap[(int64_t)b * PS]) * rl[b];
-    part[(int64_t)pch * PS + pp[PS + t]] = (a0 + a1) + (a2 + a3);
+    const double sum = (a0 + a1) + (a2 + a3);
+    part[(int64_t)pch * PS + pp[PS + t]] = pscale ? sum * (double)pscale[pch] : sum;
   }
 }
 
@@ -793,6 +836,8 @@ struct Solver {
   double *d_pA = nullptr;             // FP64 inverses (precision 0)
   float *d_pA32 = nullptr;            // FP32 inverses (precision 1)
   __hip_bfloat16 *d_pA16 = nullptr;   // BF16 inverses (precision 2)
+  __half *d_pAh = nullptr;            // FP16 inverses scaled per patch (precision 3)
+  float *d_pscale = nullptr;          // [npatch] their scale factors
   int16_t *d_pperm = nullptr;         // [npatch][2][PS] Gauss-Jordan pivot rows and their inverse
   int32_t *d_pstatus = nullptr;       // [npatch] 0, or 1 + the step of a zero pivot
   double *d_pscratch = nullptr;       // FP64 patch matrices of one chunk
@@ -1096,6 +1141,10 @@ int inner_alloc(mhd_ctx *c, int prec) {
   if (prec == 0 && !S->d_pA) MHD_HIP(hipMalloc(&S->d_pA, sizeof(double) * pelems));
   if (prec == 1 && !S->d_pA32) MHD_HIP(hipMalloc(&S->d_pA32, sizeof(float) * pelems));
   if (prec == 2 && !S->d_pA16) MHD_HIP(hipMalloc(&S->d_pA16, sizeof(__hip_bfloat16) * pelems));
+  if (prec == 3 && !S->d_pAh) {
+    MHD_HIP(hipMalloc(&S->d_pAh, sizeof(__half) * pelems));
+    MHD_HIP(hipMalloc(&S->d_pscale, sizeof(float) * std::max(S->npatch, 1)));
+  }
   if (!S->d_pscratch) {
     // the FP64 patch matrices are inverted chunk by chunk: only the inverses
     // of the solve's precision are kept for the whole mesh
@@ -1191,13 +1240,16 @@ int patch_as_on(mhd_ctx *c, const T *r, T *y, int prec, double w, bool accumulat
   Solver *S = c->solver;
   if (prec == 0)
     hipLaunchKernelGGL((k_patch_apply<double, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS, S->d_psize,
-                       S->d_pidx, S->d_pperm, S->d_pA, r, S->d_ppart);
+                       S->d_pidx, S->d_pperm, S->d_pA, r, S->d_ppart, (const float *)nullptr);
   else if (prec == 1)
     hipLaunchKernelGGL((k_patch_apply<float, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS, S->d_psize,
-                       S->d_pidx, S->d_pperm, S->d_pA32, r, S->d_ppart);
-  else
+                       S->d_pidx, S->d_pperm, S->d_pA32, r, S->d_ppart, (const float *)nullptr);
+  else if (prec == 2)
     hipLaunchKernelGGL((k_patch_apply<__hip_bfloat16, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS,
-                       S->d_psize, S->d_pidx, S->d_pperm, S->d_pA16, r, S->d_ppart);
+                       S->d_psize, S->d_pidx, S->d_pperm, S->d_pA16, r, S->d_ppart, (const float *)nullptr);
+  else
+    hipLaunchKernelGGL((k_patch_apply<__half, T>), dim3(S->npatch), dim3(256), 0, st, S->npatch, S->PS, S->d_psize,
+                       S->d_pidx, S->d_pperm, S->d_pAh, r, S->d_ppart, (const float *)S->d_pscale);
   count(c, prec == 0 ? OP_PATCH64 : prec == 1 ? OP_PATCH32 : OP_PATCH16, S->pent);
   hipLaunchKernelGGL((k_patch_gather<T>), dim3(nblocks(S->n1)), dim3(TB), 0, st, S->n1, S->d_dptr, S->d_dent,
                      S->d_ppart, w, y, accumulate ? 1 : 0);
@@ -1462,12 +1514,16 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
     double *o64 = prec == 0 ? S->d_pA + p0 * pe : nullptr;
     float *o32 = prec == 1 ? S->d_pA32 + p0 * pe : nullptr;
     __hip_bfloat16 *o16 = prec == 2 ? S->d_pA16 + p0 * pe : nullptr;
+    __half *oh = prec == 3 ? S->d_pAh + p0 * pe : nullptr;
+    float *sc = prec == 3 ? S->d_pscale + p0 : nullptr;
     if (prec == 0)
-      hipLaunchKernelGGL(k_patch_gjd<double>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, o64, pp, st);
+      hipLaunchKernelGGL(k_patch_gjd<double>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, o64, pp, st, sc);
     else if (prec == 1)
-      hipLaunchKernelGGL(k_patch_gjd<float>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, o32, pp, st);
+      hipLaunchKernelGGL(k_patch_gjd<float>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, o32, pp, st, sc);
+    else if (prec == 2)
+      hipLaunchKernelGGL(k_patch_gjd<__hip_bfloat16>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, o16, pp, st, sc);
     else
-      hipLaunchKernelGGL(k_patch_gjd<__hip_bfloat16>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, o16, pp, st);
+      hipLaunchKernelGGL(k_patch_gjd<__half>, dim3(np), dim3(GJ_T), 0, c->stream, S->PS, c->pivot_tau, mat, oh, pp, st, sc);
     MHD_HIP(hipGetLastError());
     std::vector<int32_t> status(np), flagged;
     MHD_HIP(hipMemcpyAsync(status.data(), st, sizeof(int32_t) * np, hipMemcpyDeviceToHost, c->stream));
@@ -1482,11 +1538,13 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
                            c->stream));
     const dim3 g((unsigned)flagged.size());
     if (prec == 0)
-      hipLaunchKernelGGL(k_patch_gj<double>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, o64, pp, st);
+      hipLaunchKernelGGL(k_patch_gj<double>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, o64, pp, st, sc);
     else if (prec == 1)
-      hipLaunchKernelGGL(k_patch_gj<float>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, o32, pp, st);
+      hipLaunchKernelGGL(k_patch_gj<float>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, o32, pp, st, sc);
+    else if (prec == 2)
+      hipLaunchKernelGGL(k_patch_gj<__hip_bfloat16>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, o16, pp, st, sc);
     else
-      hipLaunchKernelGGL(k_patch_gj<__hip_bfloat16>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, o16, pp, st);
+      hipLaunchKernelGGL(k_patch_gj<__half>, g, dim3(GJ_T), 0, c->stream, S->PS, d_list, mat, oh, pp, st, sc);
     MHD_HIP(hipGetLastError());
     MHD_HIP(hipMemcpyAsync(status.data(), st, sizeof(int32_t) * np, hipMemcpyDeviceToHost, c->stream));
     MHD_HIP(hipStreamSynchronize(c->stream));
@@ -1653,7 +1711,7 @@ void mhd_solver_free(mhd_ctx *c) {
   if (!c || !c->solver) return;
   Solver *S = c->solver;
   void *ptrs[] = {S->d_split, S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, S->d_psize, S->d_pidx, S->d_pA,
-                  S->d_pA32, S->d_pA16, S->d_pperm, S->d_pstatus, S->d_pscratch, S->d_val32, S->V, S->Z, S->w, S->b, S->dx, S->t1,
+                  S->d_pA32, S->d_pA16, S->d_pAh, S->d_pscale, S->d_pperm, S->d_pstatus, S->d_pscratch, S->d_val32, S->V, S->Z, S->w, S->b, S->dx, S->t1,
                   S->V1, S->Z1, S->w1, S->x1, S->r1, S->d_part, S->d_dots, S->d_y, S->d_dptr, S->d_dent,
                   S->d_ppart};
   for (void *p : ptrs)
@@ -1801,8 +1859,8 @@ void mhd_default_solver_opts(mhd_solver_opts *o) {
 
 int mhd_precond_setup(mhd_ctx *c, const mhd_solver_opts *o) {
   if (!c || !o) return -1;
-  if (o->precision < 0 || o->precision > 2) {
-    c->err = "precision must be 0, 1 or 2";
+  if (o->precision < 0 || o->precision > 3) {
+    c->err = "precision must be 0, 1, 2 or 3";
     return -1;
   }
   if (!c->jac_valid) {
@@ -1892,8 +1950,8 @@ int linear_solve(mhd_ctx *c, const double *x_state, const double *rhs, double *l
                  const mhd_solver_opts *o, mhd_solve_info *info, bool transpose) {
   if (!c || !x_state || !rhs || !lambda || !o || !info) return -1;
   std::memset(info, 0, sizeof(*info));
-  if (o->precision < 0 || o->precision > 2) {
-    c->err = "precision must be 0, 1 or 2";
+  if (o->precision < 0 || o->precision > 3) {
+    c->err = "precision must be 0, 1, 2 or 3";
     return -1;
   }
   MHD_HIP(hipSetDevice(c->device));
@@ -1980,8 +2038,9 @@ int mhd_linear_solve(mhd_ctx *c, const double *x_state, const double *rhs, doubl
 int mhd_newton_solve(mhd_ctx *c, double *x_host, const mhd_solver_opts *o, mhd_solve_info *info) {
   if (!c || !x_host || !o || !info) return -1;
   std::memset(info, 0, sizeof(*info));
-  if (o->precision < 0 || o->precision > 2) {
-    c->err = "precision must be 0 (FP64), 1 (FP32 inner, FP32 patches) or 2 (FP32 inner, BF16 patches)";
+  if (o->precision < 0 || o->precision > 3) {
+    c->err = "precision must be 0 (FP64), 1 (FP32 inner, FP32 patches), 2 (FP32 inner, BF16 patches) or 3 (FP32 "
+             "inner, FP16 patches scaled per patch)";
     return -1;
   }
   MHD_HIP(hipSetDevice(c->device));

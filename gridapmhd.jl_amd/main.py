@@ -19,7 +19,7 @@ from . import _lib, fespaces
 from .meshers import CartesianMesh
 from .parameters import add_default_params, fluid_physics
 
-PRECISION = {"fp64": 0, "mixed": 1, "mixed_bf16": 2}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_bf16": 2, "mixed_fp16": 3}
 
 
 @dataclasses.dataclass

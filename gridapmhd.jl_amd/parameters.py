@@ -75,7 +75,7 @@ def default_solver_params(name: str) -> Dict[str, Any]:
             "niter_ls": 400,      # FGMRES iterations per Newton step
             "rtol": 1e-6,
             "atol": 1e-8,
-            "precision": "fp64",  # "fp64" | "mixed" | "mixed_bf16"
+            "precision": "fp64",  # "fp64" | "mixed" | "mixed_bf16" | "mixed_fp16"
             "restart": 200,
             "inner_iters": 3,     # flexible-GMRES steps of the inexact [u,j] solve
             "levels": "auto",     # [u,j] V-cycle levels (x/y semi-coarsening); 1 = Schwarz only

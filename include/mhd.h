@@ -86,7 +86,8 @@ typedef struct {
   int32_t restart;              /* FGMRES Krylov dimension m (:niter_ls, 15) */
   int32_t max_krylov;           /* FGMRES iteration cap per Newton step */
   int32_t precision;            /* 0 FP64; 1 mixed: FP32 inner GMRES + FP32 smoother blocks inside FP64
-                                   FGMRES refinement; 2 as 1 with BF16 smoother blocks */
+                                   FGMRES refinement; 2 as 1 with BF16 smoother blocks; 3 as 1 with FP16
+                                   smoother blocks scaled by a power of two per patch */
   double rtol, atol;            /* Newton: ||R|| <= max(atol, rtol ||R0||) */
   double ls_rtol;               /* linear: ||r|| <= max(ls_rtol ||b||, max(atol, rtol ||R0||) / 2) */
   int32_t inner_iters;          /* flexible-GMRES steps of the inexact [u,j] block solve */
@@ -218,13 +219,13 @@ int mhd_attach_coarse(mhd_ctx *ctx, mhd_ctx *coarse, int64_t nrows, int64_t ncol
  * (src/Solvers/badia2024.jl:20-31) -- Schur scalings, vertex-star patch
  * extraction and batched inversion, coarse levels at the injected state.
  * opts->precision selects the storage of the patch inverses (0 FP64, 1 FP32,
- * 2 BF16).  A singular patch is an error. */
+ * 2 BF16, 3 FP16 with a per-patch power-of-two scale).  A singular patch is an error. */
 int mhd_precond_setup(mhd_ctx *ctx, const mhd_solver_opts *opts);
 
 /* y += sum_P R_P^T A_P^-1 R_P r on the [u,j] block (device pointers, length
  * nfree[u] + nfree[j]; y is accumulated into).  The smoother of the [u,j]
  * block solve, in the precision of the last mhd_precond_setup: r and y are
- * double for precision 0 and float (the inner-solve vectors) for 1 and 2.
+ * double for precision 0 and float (the inner-solve vectors) for 1 to 3.
  * The patch products of a dof are summed in FP64 in a fixed order (the same
  * result in every run) through a workspace of the context, so calls that
  * overlap each other or a solve of the same context on another stream race. */

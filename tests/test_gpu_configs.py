@@ -44,7 +44,7 @@ def _bench_opts(ctx, precision, rtol=1e-12, ls_rtol=1e-12, max_newton=8):
 
 
 @pytest.mark.parametrize("start", ["zero", "random"])
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 3])
 def test_c2_parameters_match_oracle(precision, start, orc_mod):
     import gridapmhd_amd as g
     _say(f"[configs] 64x64x8 physics on 8x8x8, bench solver options, precision={precision} start={start}")

@@ -240,7 +240,7 @@ def _vertex_patches(sp):
     return [np.array(sorted(L)) for _, L in sorted(lists.items()) if L]
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 3])
 def test_smoother_matches_patch_inverses(precision, orc_mod):
     """The vertex-star Schwarz smoother (mhd_smoother_apply_d after
     mhd_precond_setup: the patch matrices inverted on the GPU by in-register
@@ -277,4 +277,5 @@ def test_smoother_matches_patch_inverses(precision, orc_mod):
         ctx.close()
     err = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
     _say(f"  {len(patches)} patches (max {pdim} dofs): rel err {err:.2e}")
-    assert err < (1e-12 if precision == 0 else 1e-5)
+    # FP16 blocks (precision 3) carry 11 significant bits: 4.9e-4 per entry
+    assert err < {0: 1e-12, 1: 1e-5, 3: 5e-3}[precision]
