@@ -340,30 +340,33 @@ class Context:
 
 
 def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.175, coarse_step="minres",
-                        device=0) -> Context:
+                        device=0, coarsen=(0, 1)) -> Context:
     """Fine context plus `levels - 1` semi-coarsened levels attached for the
     [u,j] V-cycle (badia2024 numbering, nested x/y coarsening, the same
     boundary conditions on every level).  `levels="auto"` picks the depth
     from the mesh; `coarse_iters = 0` solves the coarsest level exactly;
-    `coarse_step` is "full" or "minres" (see mhd_attach_coarse)."""
+    `coarse_step` is "full" or "minres" (see mhd_attach_coarse); `coarsen`
+    the halved axes ((0, 1) = x and y, the reference's hierarchy; (0,) keeps y)."""
     from . import fespaces, multigrid
     if getattr(sp.mesh, "nc", None) is None:
         # unstructured (gmsh) meshes have no semi-coarsened hierarchy here
         if levels not in ("auto", 1):
             raise ValueError("the [u,j] V-cycle needs a Cartesian mesh hierarchy; use levels=1 on unstructured meshes")
         levels = 1
+    coarsen = tuple(coarsen)
     if levels == "auto":
-        levels = multigrid.auto_levels(sp.mesh)
+        levels = multigrid.auto_levels(sp.mesh, axes=coarsen)
     ctx = context_from_spaces(sp, phys, device=device)
     if levels <= 1:
         return ctx
     bc = sp.bc_spec
     # the coarse spaces and transfers depend on the mesh only: kept on the
     # fine spaces for further contexts of the same problem
-    cache = sp.__dict__.setdefault("_multigrid_cache", {"chain": [sp], "transfers": {}})
+    caches = sp.__dict__.setdefault("_multigrid_cache", {})
+    cache = caches.setdefault(coarsen, {"chain": [sp], "transfers": {}})
     chain = cache["chain"]
     while len(chain) < levels:
-        mesh = multigrid.coarsen_mesh(chain[-1].mesh)
+        mesh = multigrid.coarsen_mesh(chain[-1].mesh, axes=coarsen)
         chain.append(fespaces.setup_fe_spaces(mesh, bc["u_tags"], bc["j_tags"], bc["thin_walls"], style=sp.style))
     ctxs = [ctx] + [context_from_spaces(c, phys, device=device) for c in chain[1:levels]]
     # attach from the coarsest upwards so each attached context already carries its chain

@@ -30,15 +30,22 @@ from .meshers import CartesianMesh
 G2 = fespaces.G2
 
 
-def coarsen_mesh(mesh: CartesianMesh) -> CartesianMesh:
-    """Coarse mesh: every second node in x and y, all nodes in z."""
+def coarsen_mesh(mesh: CartesianMesh, axes=(0, 1)) -> CartesianMesh:
+    """Coarse mesh: every second node along the ``axes`` (x and y by default,
+    the reference's hierarchy; ``(0,)`` keeps y, e.g. to keep Hartmann layers
+    resolved on every level), all nodes in z."""
     nx, ny, nz = mesh.nc
-    if nx % 2 or ny % 2:
-        raise ValueError("x/y cell counts must be even to coarsen")
+    fx, fy = (2 if 0 in axes else 1), (2 if 1 in axes else 1)
+    if nx % fx or ny % fy:
+        raise ValueError("cell counts must be even along the coarsened axes")
     X = mesh.node_xyz.reshape(nz + 1, ny + 1, nx + 1, 3)
-    Xc = X[:, ::2, ::2, :].reshape(-1, 3).copy()
-    cm = CartesianMesh((nx // 2, ny // 2, nz), mesh.domain, mesh.isperiodic, Xc, dict(mesh.tags))
+    Xc = X[:, ::fy, ::fx, :].reshape(-1, 3).copy()
+    cm = CartesianMesh((nx // fx, ny // fy, nz), mesh.domain, mesh.isperiodic, Xc, dict(mesh.tags))
     return cm
+
+
+def _ratios(fine: CartesianMesh, coarse: CartesianMesh):
+    return fine.nc[0] // coarse.nc[0], fine.nc[1] // coarse.nc[1]
 
 
 def _q2_1d(t):
@@ -135,15 +142,16 @@ def _locate(coarse: CartesianMesh, fine: CartesianMesh, fine_cells: np.ndarray, 
     coarse trilinear map), which is exact on graded axis-aligned meshes and
     is the natural interpolation on non-nested (jittered) ones."""
     nx, ny, _ = fine.nc
+    rx, ry = _ratios(fine, coarse)
     fi = fine_cells % nx
     fj = (fine_cells // nx) % ny
     fk = fine_cells // (nx * ny)
     cnx, cny = coarse.nc[0], coarse.nc[1]
-    cc = fi // 2 + cnx * (fj // 2 + cny * fk)
+    cc = fi // rx + cnx * (fj // ry + cny * fk)
     X, _ = _trilinear(fine.cell_xyz()[fine_cells], fine_xi)
     guess = fine_xi.copy()
-    guess[:, 0] = 0.5 * (fine_xi[:, 0] + (fi % 2))
-    guess[:, 1] = 0.5 * (fine_xi[:, 1] + (fj % 2))
+    guess[:, 0] = (fine_xi[:, 0] + (fi % rx)) / rx
+    guess[:, 1] = (fine_xi[:, 1] + (fj % ry)) / ry
     xi = invert_map(coarse.cell_xyz()[cc], X, guess)
     return cc, xi
 
@@ -230,17 +238,18 @@ def state_interpolation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> sps.csr
     _, first = np.unique(g, return_index=True)
     cells, loc = cells[first], loc[first]
     X, _ = _trilinear(coarse.cell_xyz()[cells], fespaces.Q2_OFFSETS[loc] / 2.0)
-    # the fine cell of the coarse cell (I, J, K) holding X: fine cells (2I + a, 2J + b, K)
+    # the fine cell of the coarse cell (I, J, K) holding X: fine cells (rx I + a, ry J + b, K)
     cnx, cny = coarse.nc[0], coarse.nc[1]
     nx, ny = fine.nc[0], fine.nc[1]
+    rx, ry = _ratios(fine, coarse)
     I, J, K = cells % cnx, (cells // cnx) % cny, cells // (cnx * cny)
     best_cell = np.zeros(cells.size, dtype=np.int64)
     best_xi = np.zeros((cells.size, 3))
     best_out = np.full(cells.size, np.inf)
     fxyz = fine.cell_xyz()
-    for a in (0, 1):
-        for b in (0, 1):
-            fc = (2 * I + a) + nx * ((2 * J + b) + ny * K)
+    for a in range(rx):
+        for b in range(ry):
+            fc = (rx * I + a) + nx * ((ry * J + b) + ny * K)
             xi = invert_map(fxyz[fc], X)
             out = np.max(np.maximum(-xi, xi - 1.0), axis=1)
             take = out < best_out - 1e-12
@@ -261,14 +270,15 @@ def state_interpolation(sp_f: fespaces.Spaces, sp_c: fespaces.Spaces) -> sps.csr
     return R
 
 
-def auto_levels(mesh: CartesianMesh, min_cells: int = 136, min_xy: int = 8) -> int:
-    """Number of levels: halve x and y while both are even and >= `min_xy` and
-    the level still has more than `min_cells` cells (about 6,000 [u, j] dofs,
-    small enough for the exact coarsest solve)."""
-    nx, ny, nz = mesh.nc
+def auto_levels(mesh: CartesianMesh, min_cells: int = 136, min_xy: int = 8, axes=(0, 1)) -> int:
+    """Number of levels: halve the ``axes`` while their cell counts are even
+    and >= `min_xy` and the level still has more than `min_cells` cells (about
+    6,000 [u, j] dofs, small enough for the exact coarsest solve)."""
+    n = list(mesh.nc)
     levels = 1
-    while nx % 2 == 0 and ny % 2 == 0 and nx >= min_xy and ny >= min_xy and nx * ny * nz > min_cells:
-        nx, ny = nx // 2, ny // 2
+    while all(n[a] % 2 == 0 and n[a] >= min_xy for a in axes) and n[0] * n[1] * n[2] > min_cells:
+        for a in axes:
+            n[a] //= 2
         levels += 1
     return levels
 

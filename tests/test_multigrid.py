@@ -39,12 +39,15 @@ def _j(X, zw=1.0):
                      z * z + 0.6 * x * y - 0.1 * z * y], 1)
 
 
+@pytest.mark.parametrize("axes", [(0, 1), (0,)])
 @pytest.mark.parametrize("kind", ["uniform", "tanh3", "tanh2_xy", "hunt_smolentsev"])
-def test_prolongation_reproduces_coarse_fields(kind):
+def test_prolongation_reproduces_coarse_fields(kind, axes):
+    """Both hierarchies: x and y halved (the reference's), and x only."""
     import gridapmhd_amd as g
     fine = _mesh(kind)
     g.meshers.hunt_add_tags(fine)
-    coarse = g.multigrid.coarsen_mesh(fine)
+    coarse = g.multigrid.coarsen_mesh(fine, axes=axes)
+    assert coarse.nc == (fine.nc[0] // 2, fine.nc[1] // (2 if 1 in axes else 1), fine.nc[2])
     spf = g.fespaces.setup_fe_spaces(fine, [], [], style="badia2024")
     spc = g.fespaces.setup_fe_spaces(coarse, [], [], style="badia2024")
     P, R = g.multigrid.prolongation(spf, spc)
