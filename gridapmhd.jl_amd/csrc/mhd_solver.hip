@@ -174,6 +174,67 @@ __global__ __launch_bounds__(256) void k_spmv_seg(int64_t nrows, const int64_t *
   }
 }
 
+// Rows that share one column list (the 3 components of a velocity node, the 4
+// flux dofs of a face, the 12 interior current dofs, the 4 pressure and 8
+// potential dofs of a cell) form a group: its columns are stored once, its
+// values stay where the CSR has them (row i of the group at rowptr[r0] +
+// i * len).  `LPG` lanes per group; each column's x value is loaded once for
+// all rows of the group.  MODE 0: y = A x (full rows); 1: y = A11 x (the
+// prefix up to split); 3: y = v - A11 x.
+constexpr int GMAX = 12;
+template <class MV, class TX, class TY, int LPG, int MODE>
+__global__ __launch_bounds__(256) void k_spmv_grp(int64_t ngroups, const int64_t *__restrict__ grow,
+                                                  const int64_t *__restrict__ gcp, const int32_t *__restrict__ gcol,
+                                                  const int64_t *__restrict__ rowptr, const int64_t *__restrict__ split,
+                                                  const MV *__restrict__ val, const TX *__restrict__ x,
+                                                  const TY *__restrict__ v, TY *__restrict__ y) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t g = gid / LPG;
+  const int lane = threadIdx.x % LPG;
+  double acc[GMAX];
+#pragma unroll
+  for (int i = 0; i < GMAX; ++i) acc[i] = 0.0;
+  int m = 0;
+  int64_t r0 = 0;
+  if (g < ngroups) {
+    r0 = grow[g];
+    m = (int)(grow[g + 1] - r0);
+    const int64_t b = rowptr[r0];
+    const int64_t len = rowptr[r0 + 1] - b;
+    const int64_t e = MODE == 0 ? len : split[r0] - b;
+    const int32_t *__restrict__ cols = gcol + gcp[g];
+    const MV *__restrict__ vb = val + b;
+    int64_t k = lane;
+    // four columns per step: four independent column -> x load chains in flight
+    for (; k + 3 * LPG < e; k += 4 * LPG) {
+      const double x0 = (double)x[cols[k]], x1 = (double)x[cols[k + LPG]];
+      const double x2 = (double)x[cols[k + 2 * LPG]], x3 = (double)x[cols[k + 3 * LPG]];
+#pragma unroll
+      for (int i = 0; i < GMAX; ++i)
+        if (i < m) {
+          const MV *__restrict__ vr = vb + i * len + k;
+          acc[i] += ((double)vr[0] * x0 + (double)vr[LPG] * x1) + ((double)vr[2 * LPG] * x2 + (double)vr[3 * LPG] * x3);
+        }
+    }
+    for (; k < e; k += LPG) {
+      const double xv = (double)x[cols[k]];
+#pragma unroll
+      for (int i = 0; i < GMAX; ++i)
+        if (i < m) acc[i] += (double)vb[i * len + k] * xv;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < GMAX; ++i) {
+    double sm = acc[i];
+#pragma unroll
+    for (int off = LPG / 2; off > 0; off >>= 1) sm += __shfl_down(sm, off, LPG);
+    if (lane == 0 && i < m) {
+      const int64_t row = r0 + i;
+      y[row] = MODE == 3 ? (TY)((double)v[row] - sm) : (TY)sm;
+    }
+  }
+}
+
 // y (+)= A x for a plain CSR matrix (prolongation / restriction)
 template <class MV, class T, int LPR, bool ACC>
 __global__ __launch_bounds__(256) void k_csr(int64_t nrows, const int64_t *__restrict__ rowptr,
@@ -861,6 +922,11 @@ struct Solver {
   int64_t nnz11 = 0, nnz1x = 0;
   double pent = 0.0, pcube = 0.0;
   double ops[MHD_NOPS] = {};
+  // row groups sharing a column list (k_spmv_grp): all rows, and the first
+  // ngroups11 groups cover the [u,j] rows exactly
+  int64_t ngroups = 0, ngroups11 = 0;
+  int64_t *d_grow = nullptr, *d_gcp = nullptr;
+  int32_t *d_gcol = nullptr;
 };
 
 namespace {
@@ -1031,6 +1097,36 @@ int solver_init(mhd_ctx *c, const mhd_solver_opts *o) {
     const int64_t k = std::lower_bound(b, e, (int32_t)S->n1) - b;
     S->nnz11 += k;
     S->nnz1x += (e - b) - k;
+  }
+  {
+    // runs of consecutive rows with one column list (at most GMAX rows; no run
+    // crosses the end of the [u,j] rows)
+    std::vector<int64_t> grow, gcp;
+    std::vector<int32_t> gcol;
+    const int64_t *rp = c->h_rowptr.data();
+    const int32_t *ci = c->h_colind.data();
+    int64_t r = 0;
+    while (r < S->n) {
+      const int64_t len = rp[r + 1] - rp[r];
+      int64_t m = 1;
+      while (r + m < S->n && m < GMAX && (r + m != S->n1) && rp[r + m + 1] - rp[r + m] == len &&
+             std::equal(ci + rp[r], ci + rp[r] + len, ci + rp[r + m]))
+        ++m;
+      grow.push_back(r);
+      gcp.push_back((int64_t)gcol.size());
+      gcol.insert(gcol.end(), ci + rp[r], ci + rp[r] + len);
+      if (r < S->n1) S->ngroups11 = (int64_t)grow.size();
+      r += m;
+    }
+    grow.push_back(S->n);
+    gcp.push_back((int64_t)gcol.size());
+    S->ngroups = (int64_t)grow.size() - 1;
+    if ((rc = dmalloc(c, &S->d_grow, grow.size()))) return rc;
+    if ((rc = dmalloc(c, &S->d_gcp, gcp.size()))) return rc;
+    if ((rc = dmalloc(c, &S->d_gcol, gcol.size()))) return rc;
+    MHD_HIP(hipMemcpy(S->d_grow, grow.data(), sizeof(int64_t) * grow.size(), hipMemcpyHostToDevice));
+    MHD_HIP(hipMemcpy(S->d_gcp, gcp.data(), sizeof(int64_t) * gcp.size(), hipMemcpyHostToDevice));
+    MHD_HIP(hipMemcpy(S->d_gcol, gcol.data(), sizeof(int32_t) * gcol.size(), hipMemcpyHostToDevice));
   }
   hipLaunchKernelGGL(k_split, dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, c->d_rowptr, c->d_colind,
                      S->d_split);
@@ -1262,20 +1358,40 @@ int patch_as(mhd_ctx *c, const T *r, T *y, int prec, double w = 1.0, bool accumu
   return patch_as_on<T>(c, r, y, prec, w, accumulate, c->stream);
 }
 
-template <class T>
-int spmv_a11(mhd_ctx *c, const T *x, T *y, int prec) {
+// y = A11 x (MODE 1) or y = v - A11 x (MODE 3) on the [u,j] rows, FP64 values
+// for precision 0 and the FP32 copy otherwise, through the row groups
+template <class T, int MODE>
+int a11_product(mhd_ctx *c, const T *x, const T *v, T *y, int prec) {
   Solver *S = c->solver;
-  constexpr int LPR = 16;
-  const int grid = (int)((S->n1 * LPR + 255) / 256);
+  constexpr int LPG = 16;
+  const int grid = (int)((S->ngroups11 * LPG + 255) / 256);
   if (prec == 0)
-    hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 1>), dim3(grid), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
-                       S->d_split, c->d_colind, c->d_val, x, (const T *)nullptr, y);
+    hipLaunchKernelGGL((k_spmv_grp<double, T, T, LPG, MODE>), dim3(grid), dim3(256), 0, c->stream, S->ngroups11,
+                       S->d_grow, S->d_gcp, S->d_gcol, c->d_rowptr, S->d_split, c->d_val, x, v, y);
   else
-    hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 1>), dim3(grid), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
-                       S->d_split, c->d_colind, S->d_val32, x, (const T *)nullptr, y);
+    hipLaunchKernelGGL((k_spmv_grp<float, T, T, LPG, MODE>), dim3(grid), dim3(256), 0, c->stream, S->ngroups11,
+                       S->d_grow, S->d_gcp, S->d_gcol, c->d_rowptr, S->d_split, S->d_val32, x, v, y);
   count(c, prec == 0 ? OP_SPMV64 : OP_SPMV32, (double)S->nnz11);
   MHD_HIP(hipGetLastError());
   return 0;
+}
+
+// y = J x with the full FP64 Jacobian of the context, through the row groups
+int full_product(mhd_ctx *c, const double *x, double *y) {
+  Solver *S = c->solver;
+  constexpr int LPG = 16;
+  const int grid = (int)((S->ngroups * LPG + 255) / 256);
+  hipLaunchKernelGGL((k_spmv_grp<double, double, double, LPG, 0>), dim3(grid), dim3(256), 0, c->stream, S->ngroups,
+                     S->d_grow, S->d_gcp, S->d_gcol, c->d_rowptr, S->d_split, c->d_val, x, (const double *)nullptr,
+                     y);
+  count(c, OP_SPMV64, (double)c->nnz);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class T>
+int spmv_a11(mhd_ctx *c, const T *x, T *y, int prec) {
+  return a11_product<T, 1>(c, x, (const T *)nullptr, y, prec);
 }
 
 template <class T>
@@ -1321,12 +1437,7 @@ int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
   T *rcv = (T *)L->rc, *ec = (T *)L->ec;
   const int gfine = (int)((S->n1 * LPR + 255) / 256), gc = (int)((L->n1c * LPR + 255) / 256);
   // rr = r - A11 y
-  if (prec == 0)
-    hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
-                       S->d_split, c->d_colind, c->d_val, y, r, rr);
-  else
-    hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
-                       S->d_split, c->d_colind, S->d_val32, y, r, rr);
+  if ((rc = a11_product<T, 3>(c, y, r, rr, prec))) return rc;
   // restriction
   if (prec == 0)
     hipLaunchKernelGGL((k_csr<double, T, LPR, false>), dim3(gc), dim3(256), 0, c->stream, L->n1c, L->d_Trow, L->d_Tcol,
@@ -1335,8 +1446,8 @@ int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
     hipLaunchKernelGGL((k_csr<float, T, LPR, false>), dim3(gc), dim3(256), 0, c->stream, L->n1c, L->d_Trow, L->d_Tcol,
                        L->d_Tval32, rr, rcv);
   MHD_HIP(hipGetLastError());
-  // residual SpMV, restriction, prolongation, and the second SpMV of either coarse step
-  count(c, prec == 0 ? OP_SPMV64 : OP_SPMV32, 2.0 * S->nnz11 + 2.0 * L->nnzP);
+  // restriction and prolongation (the A11 products count themselves)
+  count(c, prec == 0 ? OP_SPMV64 : OP_SPMV32, 2.0 * L->nnzP);
   if (cc->coarse) {
     if ((rc = vcycle<T>(cc, rcv, ec, prec))) return rc;
   } else if (L->coarse_iters == 0) {
@@ -1347,33 +1458,25 @@ int vcycle(mhd_ctx *c, const T *r, T *y, int prec) {
     MHD_HIP(hipMemcpyAsync(ec, SC->x1, sizeof(T) * L->n1c, hipMemcpyDeviceToDevice, c->stream));
   }
   if (L->step_mode == 0) {  // y += P ec, rr = r - A11 y
-    if (prec == 0) {
+    if (prec == 0)
       hipLaunchKernelGGL((k_csr<double, T, LPR, true>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow,
                          L->d_Pcol, L->d_Pval, ec, y);
-      hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1,
-                         c->d_rowptr, S->d_split, c->d_colind, c->d_val, y, r, rr);
-    } else {
+    else
       hipLaunchKernelGGL((k_csr<float, T, LPR, true>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow,
                          L->d_Pcol, L->d_Pval32, ec, y);
-      hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 3>), dim3(gfine), dim3(256), 0, c->stream, S->n1,
-                         c->d_rowptr, S->d_split, c->d_colind, S->d_val32, y, r, rr);
-    }
     MHD_HIP(hipGetLastError());
+    if ((rc = a11_product<T, 3>(c, y, r, rr, prec))) return rc;
     return patch_as<T>(c, rr, y, prec, L->omega, true);
   }
   // yc = P ec, w = A11 yc
-  if (prec == 0) {
+  if (prec == 0)
     hipLaunchKernelGGL((k_csr<double, T, LPR, false>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow,
                        L->d_Pcol, L->d_Pval, ec, yc);
-    hipLaunchKernelGGL((k_spmv_seg<double, T, T, LPR, 1>), dim3(gfine), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
-                       S->d_split, c->d_colind, c->d_val, yc, (const T *)nullptr, wv);
-  } else {
+  else
     hipLaunchKernelGGL((k_csr<float, T, LPR, false>), dim3(gfine), dim3(256), 0, c->stream, S->n1, L->d_Prow,
                        L->d_Pcol, L->d_Pval32, ec, yc);
-    hipLaunchKernelGGL((k_spmv_seg<float, T, T, LPR, 1>), dim3(gfine), dim3(256), 0, c->stream, S->n1, c->d_rowptr,
-                       S->d_split, c->d_colind, S->d_val32, yc, (const T *)nullptr, wv);
-  }
   MHD_HIP(hipGetLastError());
+  if ((rc = a11_product<T, 1>(c, yc, (const T *)nullptr, wv, prec))) return rc;
   // (rr . w, w . w) on the device; y += a yc, rr -= a w
   if ((rc = mdot(c, rr, ld, 2, wv, S->n1, L->d_ab))) return rc;
   hipLaunchKernelGGL((k_cc_update<T>), dim3(nblocks(S->n1)), dim3(TB), 0, c->stream, S->n1, L->d_ab, yc, wv, y, rr);
@@ -1649,8 +1752,7 @@ int fgmres(mhd_ctx *c, const double *b, double *x, double tol, int maxit, int pr
     double est = beta;
     for (int j = 0; j < m && it < maxit; ++j) {
       if ((rc = precond(c, S->V + j * n, S->Z + j * n, prec))) return rc;
-      if ((rc = mhd_launch_spmv(c, c->d_val, S->Z + j * n, S->w, c->stream))) return rc;
-      count(c, OP_SPMV64, (double)c->nnz);
+      if ((rc = full_product(c, S->Z + j * n, S->w))) return rc;
       count(c, OP_OUTER, 1.0);
       double nrm;
       if ((rc = arnoldi<double>(c, S->V, n, j + 1, S->w, n, h.data(), &nrm))) return rc;
@@ -1688,8 +1790,7 @@ int fgmres(mhd_ctx *c, const double *b, double *x, double tol, int maxit, int pr
                        S->d_y, 1.0, x, n);
     MHD_HIP(hipGetLastError());
     // true residual for the restart: w = b - J x
-    if ((rc = mhd_launch_spmv(c, c->d_val, x, S->w, c->stream))) return rc;
-    count(c, OP_SPMV64, (double)c->nnz);
+    if ((rc = full_product(c, x, S->w))) return rc;
     count(c, OP_VEC, (k + 2.0) * n * sizeof(double));
     hipLaunchKernelGGL((k_scale_copy<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, -1.0, S->w, S->w);
     hipLaunchKernelGGL((k_axpy<double>), dim3(nblocks(n)), dim3(TB), 0, c->stream, n, 1.0, b, S->w);
@@ -1713,7 +1814,7 @@ void mhd_solver_free(mhd_ctx *c) {
   void *ptrs[] = {S->d_split, S->d_pfirst, S->d_phifirst, S->d_Sp, S->d_Sphi, S->d_psize, S->d_pidx, S->d_pA,
                   S->d_pA32, S->d_pA16, S->d_pAh, S->d_pscale, S->d_pperm, S->d_pstatus, S->d_pscratch, S->d_val32, S->V, S->Z, S->w, S->b, S->dx, S->t1,
                   S->V1, S->Z1, S->w1, S->x1, S->r1, S->d_part, S->d_dots, S->d_y, S->d_dptr, S->d_dent,
-                  S->d_ppart};
+                  S->d_ppart, S->d_grow, S->d_gcp, S->d_gcol};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (S->h_dots) (void)hipHostFree(S->h_dots);
