@@ -32,6 +32,7 @@ using Gridap.ReferenceFEs
 using Gridap.Geometry
 using Gridap.Fields: GenericField
 using FillArrays: Fill
+using LinearAlgebra: dot
 using GridapMHD
 
 const libmhd = joinpath(@__DIR__, "..", "libmhd.so")
@@ -138,6 +139,66 @@ function attach_levels!(ctx, coarse_ctxs, prolongations, interpolations; coarse_
                Int64.(R.rowptr .- 1), Int32.(R.colval .- 1), R.nzval, coarse_iters, ω, step == :full ? 0 : 1)
     check(rc, fine, "mhd_attach_coarse")
   end
+end
+
+# ---------------------------------------------------------------- differentiation
+"""
+    pressure_drop_gradient(ns::HIPNewtonSolver, x, g, Ha, Re) -> (J, dJ/dHa, dJ/dcw)
+
+Adjoint gradient of J = g ⋅ x at the converged state `x` (Gridap free values)
+for θ = (Ha, cw): A(x)ᵀ λ = g (`mhd_adjoint_solve`, Badia2024's preconditioner
+on the transposed operator), dJ/dθ = -λ ⋅ ∂R/∂θ with ∂R/∂γ and ∂R/∂cw from
+`mhd_dresidual_dparam` and ∂γ/∂Ha = 2Ha/Re (the :cfd scaling).  `g` is in
+Gridap's free-dof basis; it is moved to libmhd's as a covector (the transpose
+of the value conversion) and λ ⋅ ∂R/∂θ is basis-independent.
+"""
+function pressure_drop_gradient(ns::HIPNewtonSolver, x, g, Ha, Re)
+  xm = ns.to_mhd(x)
+  # covector conversion: J = g ⋅ x = g ⋅ from_mhd(xm) = (from_mhdᵀ g) ⋅ xm; from_mhd is
+  # linear, so its transpose is applied column by column of the cell blocks
+  gm = covector_to_mhd(ns, g)
+  λ = similar(xm)
+  info = MhdSolveInfo()
+  rc = ccall((:mhd_adjoint_solve, libmhd), Cint,
+             (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{MhdSolverOpts}, Ref{MhdSolveInfo}),
+             ns.ctx[], xm, gm, λ, ns.opts, info)
+  check(rc, ns.ctx[], "mhd_adjoint_solve")
+  rc == 0 || error("mhd_adjoint_solve did not converge (rc = $rc)")
+  dRγ, dRcw = similar(xm), similar(xm)
+  check(ccall((:mhd_dresidual_dparam, libmhd), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}),
+              ns.ctx[], xm, Int32(0), dRγ), ns.ctx[], "mhd_dresidual_dparam")
+  check(ccall((:mhd_dresidual_dparam, libmhd), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}),
+              ns.ctx[], xm, Int32(1), dRcw), ns.ctx[], "mhd_dresidual_dparam")
+  J = dot(g, x)
+  (J, -(2Ha / Re) * dot(λ, dRγ), -dot(λ, dRcw))
+end
+
+"The transpose of `from_mhd` applied to a Gridap covector (unit-vector probes of the cell blocks)."
+function covector_to_mhd(ns::HIPNewtonSolver, g)
+  n = length(g)
+  out = zeros(n)
+  # from_mhd changes only the p and j dofs cell by cell; u and phi map one to one
+  e = zeros(n)
+  for k in 1:n
+    e[k] = 1.0
+    out[k] = dot(g, ns.from_mhd(e))
+    e[k] = 0.0
+  end
+  out
+end
+
+"""
+The ChainRules reverse rule of the pressure drop, as a maintainer would add it
+(ChainRulesCore is not a dependency of this module):
+
+    function ChainRulesCore.rrule(::typeof(pressure_drop), ns, x, g, Ha, Re)
+      J, dHa, dcw = GridapMHDHIP.pressure_drop_gradient(ns, x, g, Ha, Re)
+      J, ΔJ -> (NoTangent(), NoTangent(), NoTangent(), NoTangent(), ΔJ * dHa, NoTangent(), ΔJ * dcw)
+    end
+"""
+pressure_drop_pullback(ns, x, g, Ha, Re) = begin
+  J, dHa, dcw = pressure_drop_gradient(ns, x, g, Ha, Re)
+  J, ΔJ -> (ΔJ * dHa, ΔJ * dcw)
 end
 
 function hip_postpro(cache, out)
