@@ -90,6 +90,7 @@ mutable struct HIPNewtonSolver <: Gridap.Algebra.NonlinearSolver
   opts::MhdSolverOpts
   to_mhd::Function      # Gridap free values -> libmhd free values
   from_mhd::Function    # libmhd free values -> Gridap free values
+  covec_to_mhd::Function  # a Gridap covector -> libmhd's (transpose of from_mhd's linear part)
   keep::Vector{Any}
 end
 
@@ -105,7 +106,7 @@ function GridapMHD._solver(::Val{:hip}, op, params)
   prec = Dict(:fp64 => 0, :mixed => 1, :mixed_bf16 => 2)[s[:precision]]
   opts = MhdSolverOpts(s[:niter], s[:restart], s[:niter_ls], prec, s[:rtol], s[:atol], s[:rtol] / 10,
                        s[:inner_iters], 0)
-  solver = HIPNewtonSolver(ctx, opts, data.to_mhd, data.from_mhd, data.keep)
+  solver = HIPNewtonSolver(ctx, opts, data.to_mhd, data.from_mhd, data.covec_to_mhd, data.keep)
   finalizer(x -> ccall((:mhd_destroy, libmhd), Cvoid, (Ptr{Cvoid},), x.ctx[]), solver)
   solver
 end
@@ -156,7 +157,7 @@ function pressure_drop_gradient(ns::HIPNewtonSolver, x, g, Ha, Re)
   xm = ns.to_mhd(x)
   # covector conversion: J = g ⋅ x = g ⋅ from_mhd(xm) = (from_mhdᵀ g) ⋅ xm; from_mhd is
   # linear, so its transpose is applied column by column of the cell blocks
-  gm = covector_to_mhd(ns, g)
+  gm = ns.covec_to_mhd(g)
   λ = similar(xm)
   info = MhdSolveInfo()
   rc = ccall((:mhd_adjoint_solve, libmhd), Cint,
@@ -171,20 +172,6 @@ function pressure_drop_gradient(ns::HIPNewtonSolver, x, g, Ha, Re)
               ns.ctx[], xm, Int32(1), dRcw), ns.ctx[], "mhd_dresidual_dparam")
   J = dot(g, x)
   (J, -(2Ha / Re) * dot(λ, dRγ), -dot(λ, dRcw))
-end
-
-"The transpose of `from_mhd` applied to a Gridap covector (unit-vector probes of the cell blocks)."
-function covector_to_mhd(ns::HIPNewtonSolver, g)
-  n = length(g)
-  out = zeros(n)
-  # from_mhd changes only the p and j dofs cell by cell; u and phi map one to one
-  e = zeros(n)
-  for k in 1:n
-    e[k] = 1.0
-    out[k] = dot(g, ns.from_mhd(e))
-    e[k] = 0.0
-  end
-  out
 end
 
 """
@@ -349,6 +336,35 @@ function extract_problem(U, params)
   end
   to_mhd(x) = convert(x, Mpi, Mji)
   from_mhd(x) = convert(x, Mp, Mj)
+  # the transpose of from_mhd's linear part: u and phi map one to one, p
+  # through Mpᵀ per cell, j through s Mjᵀ s per cell, each Gridap face dof
+  # taken from the first cell that produces it (every cell produces the same value)
+  function covec_to_mhd(g)
+    gm = copy(g)
+    for c in 1:ncells
+      gp = [off[:p] + k for k in ids_p[c]]
+      gm[gp] = Mp' * g[gp]
+    end
+    jr = off[:j] .+ (1:nfree[3])
+    gm[jr] .= 0.0
+    seen = falses(nfree[3])
+    for c in 1:ncells
+      gj = ids_j[c]
+      s = [i <= 24 ? flip[c][i] : 1.0 for i in 1:36]
+      w = zeros(36)
+      for (i, k) in enumerate(gj)
+        if k > 0 && !seen[k]
+          w[i] = g[off[:j] + k]
+          seen[k] = true
+        end
+      end
+      t = s .* (Mj' * (s .* w))
+      for (i, k) in enumerate(gj)
+        k > 0 && (gm[off[:j] + k] += t[i])
+      end
+    end
+    gm
+  end
   # thin walls: boundary faces of the tags, as (cell, local face)
   tw_cell, tw_lf, tw_tau, tw_cw, tw_jw = Int32[], Int8[], Float64[], Float64[], Float64[]
   if haskey(params[:bcs], :thin_wall)
@@ -380,7 +396,8 @@ function extract_problem(U, params)
                  pointer(tw_cell), pointer(tw_lf), pointer(tw_tau), pointer(tw_cw), pointer(tw_jw))
   spaces = MhdSpaces(Int32.((nfree[1], nfree[2], nfree[3], nfree[4])), Int32(length(dir_vals)), Int32(1),
                      pointer(cell_dofs), pointer(rt_sign), pointer(dir_vals))
-  (mesh = mesh, spaces = spaces, params = mparams, to_mhd = to_mhd, from_mhd = from_mhd, keep = keep)
+  (mesh = mesh, spaces = spaces, params = mparams, to_mhd = to_mhd, from_mhd = from_mhd,
+   covec_to_mhd = covec_to_mhd, keep = keep)
 end
 
 end # module
