@@ -104,7 +104,11 @@ def test_hunt_at_scale_matches_reference_norms(cx, Ha, precision):
         rel = abs(info[k] / ref[k] - 1.0)
         _say(f"  {k}: {info[k]:.15g} vs {ref[k]:.15g} rel {rel:.2e}")
         assert rel < 1e-9, (k, rel)
-    assert info["divj_l2"] < 1e-9 * info["jh_l2"]
+    # discrete div j_h vanishes with the phi-equation residual: the Newton stop
+    # (1e-11 of the initial residual) leaves it at 1e-9 .. 2e-9 of |j_h|
+    # depending on the Krylov path (measured, Ha = 500 mixed)
+    _say(f"  divj_l2 / jh_l2 = {info['divj_l2'] / info['jh_l2']:.2e}")
+    assert info["divj_l2"] < 1e-8 * info["jh_l2"]
 
 
 def test_c2_full_size_properties(orc_mod):
