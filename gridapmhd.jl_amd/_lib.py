@@ -25,6 +25,11 @@ class MhdMesh(ctypes.Structure):
                 ("tw_cw", c_void_p), ("tw_jw", c_void_p)]
 
 
+class MhdSkeleton(ctypes.Structure):
+    _fields_ = [("nfaces", c_int32), ("pad_", c_int32), ("face_cell", c_void_p), ("face_lface", c_void_p),
+                ("face_perm", c_void_p), ("mu", c_void_p)]
+
+
 class MhdSpaces(ctypes.Structure):
     _fields_ = [("nfree", c_int32 * 4), ("ndir", c_int32), ("block_style", c_int32), ("cell_dofs", c_void_p),
                 ("rt_sign", c_void_p), ("dir_vals", c_void_p)]
@@ -53,6 +58,9 @@ class MhdSolveInfo(ctypes.Structure):
 EXPORTS = {
     "mhd_create": (ctypes.c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(MhdMesh), ctypes.POINTER(MhdSpaces),
                                   ctypes.POINTER(MhdParams), c_int32]),
+    "mhd_create_stabilized": (ctypes.c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(MhdMesh),
+                                             ctypes.POINTER(MhdSpaces), ctypes.POINTER(MhdParams),
+                                             ctypes.POINTER(MhdSkeleton), c_int32]),
     "mhd_destroy": (None, [c_void_p]),
     "mhd_last_error": (ctypes.c_char_p, [c_void_p]),
     "mhd_set_params": (ctypes.c_int, [c_void_p, ctypes.POINTER(MhdParams)]),
@@ -142,7 +150,7 @@ class Context:
     """One libmhd context: the operator of one problem on one device."""
 
     def __init__(self, cell_xyz, cell_dofs, rt_sign, nfree4, dir_vals, phys, faces=None, device=0,
-                 cell_verts=None, block_style=0):
+                 cell_verts=None, block_style=0, skeleton=None):
         self.lib = load()
         self._keep = []
         k = lambda a, dt: self._keep.append(np.ascontiguousarray(a, dtype=dt)) or self._keep[-1]
@@ -167,7 +175,16 @@ class Context:
         s.cell_dofs, s.rt_sign, s.dir_vals = _ptr(dofs), _ptr(sign), _ptr(dv)
         self.params = make_params(phys)
         h = c_void_p()
-        rc = self.lib.mhd_create(ctypes.byref(h), ctypes.byref(m), ctypes.byref(s), ctypes.byref(self.params), device)
+        if skeleton is not None:
+            # skeleton = dict(cell (nf, 2), lface (nf, 2), perm (nf, 9), mu (nf,))
+            sc = k(skeleton["cell"], np.int32)
+            sk = MhdSkeleton(nfaces=sc.shape[0], face_cell=_ptr(sc), face_lface=_ptr(k(skeleton["lface"], np.int8)),
+                             face_perm=_ptr(k(skeleton["perm"], np.int8)), mu=_ptr(k(skeleton["mu"], np.float64)))
+            rc = self.lib.mhd_create_stabilized(ctypes.byref(h), ctypes.byref(m), ctypes.byref(s),
+                                                ctypes.byref(self.params), ctypes.byref(sk), device)
+        else:
+            rc = self.lib.mhd_create(ctypes.byref(h), ctypes.byref(m), ctypes.byref(s), ctypes.byref(self.params),
+                                     device)
         self.h = h
         if rc != 0:
             msg = self.lib.mhd_last_error(h).decode() if h.value else "allocation failed"
@@ -379,7 +396,9 @@ def context_with_levels(sp, phys, levels=1, coarse_iters=0, smoother_weight=0.17
 
 
 def context_from_spaces(sp, phys, dir_vals=None, device=0) -> Context:
-    """Context for a ``fespaces.Spaces`` object (thin-wall faces concatenated)."""
+    """Context for a ``fespaces.Spaces`` object (thin-wall faces concatenated).
+    ``phys["stab_mu"]`` > 0 adds the skeleton stabilisation with that mu on
+    every interior face."""
     tw = sp.thin_wall_faces
     faces = None
     if tw:
@@ -389,5 +408,11 @@ def context_from_spaces(sp, phys, dir_vals=None, device=0) -> Context:
                      jw=np.concatenate([np.full(t["cell"].size, t["jw"]) for t in tw]))
     nf = [sp.nfree["u"], sp.nfree["p"], sp.nfree["j"], sp.nfree["phi"]]
     dv = sp.dir_vals if dir_vals is None else dir_vals
+    skel = None
+    mu = float(phys.get("stab_mu", 0.0))
+    if mu != 0.0:
+        from . import fespaces
+        skel = dict(fespaces.skeleton_faces(sp))
+        skel["mu"] = np.full(skel["cell"].shape[0], mu)
     return Context(sp.mesh.cell_xyz(), sp.cell_dofs, sp.rt_sign, nf, dv, phys, faces, device,
-                   cell_verts=sp.cell_verts, block_style=1 if sp.style == "badia2024" else 0)
+                   cell_verts=sp.cell_verts, block_style=1 if sp.style == "badia2024" else 0, skeleton=skel)

@@ -25,7 +25,7 @@ from .main import main
 def hunt(nc=(4, 4), nz=3, ν=1.0, ρ=1.0, σ=1.0, B=(0.0, 10.0, 0.0), f=(0.0, 0.0, 1.0), ζᵤ=0.0, ζⱼ=0.0, L=1.0,
          u0=1.0, B0=None, order=2, order_j=None, formulation="cfd", convection="none", BL_adapted=True,
          kmap_x=1, kmap_y=1, solver="hip", initial_value="zero", domain=(-1.0, 1.0, -1.0, 1.0, 0.0, 0.1),
-         debug=False, solve=True, res_assemble=False, jac_assemble=False, nsums=10, errors=True):
+         debug=False, solve=True, res_assemble=False, jac_assemble=False, nsums=10, errors=True, μ=0.0):
     B0 = float(np.linalg.norm(B)) if B0 is None else B0
     order_j = order if order_j is None else order_j
     info = {}
@@ -53,6 +53,9 @@ def hunt(nc=(4, 4), nz=3, ν=1.0, ρ=1.0, σ=1.0, B=(0.0, 10.0, 0.0), f=(0.0, 0.
         "solver": solver,
         "x0": initial_value,
     }
+    if μ > 0:
+        # skeleton stabilisation weight (src/Applications/hunt.jl:52, 191-193)
+        params["bcs"]["stabilization"] = {"μ": μ}
     xh, fullparams, out = main(params, output=info)
     ci = out["cell_integrals"].sum(axis=0)
     # rescale: uh = u0 ubar, jh = sigma u0 B0 jbar (src/Applications/hunt.jl:213-218)

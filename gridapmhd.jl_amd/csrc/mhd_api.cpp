@@ -1,6 +1,7 @@
 // C-ABI entry points of libmhd (declared in include/mhd.h).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -51,8 +52,8 @@ extern "C" {
 
 const char *mhd_last_error(const mhd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, const mhd_params *params,
-               int32_t device) {
+int mhd_create_stabilized(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, const mhd_params *params,
+                          const mhd_skeleton *skel, int32_t device) {
   if (!out || !mesh || !spaces || !params) return -1;
   *out = nullptr;
   mhd_ctx *c = new (std::nothrow) mhd_ctx();
@@ -106,6 +107,82 @@ int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, co
     c->err = "spaces: dir_vals missing";
     return -1;
   }
+  // skeleton faces: slot dofs of the two cells' velocity nodes, checked on the host
+  std::vector<int32_t> sdofs;
+  std::vector<uint8_t> mslot;
+  const int32_t nskel = skel ? skel->nfaces : 0;
+  if (nskel < 0 || (nskel > 0 && (!skel->face_cell || !skel->face_lface || !skel->face_perm || !skel->mu))) {
+    c->err = "skeleton: face arrays missing";
+    return -1;
+  }
+  if ((int64_t)nskel * mhd::SKEL_DOFS >= ((int64_t)1 << 31)) {
+    c->err = "skeleton: too many faces for the int32 entry index";
+    return -1;
+  }
+  if (nskel > 0) {
+    sdofs.assign((size_t)nskel * mhd::SKEL_DOFS, mhd::SKEL_NONE);
+    mslot.assign((size_t)nskel * 27, 0);
+    for (int32_t f = 0; f < nskel; ++f) {
+      const std::string where = "skeleton face " + std::to_string(f) + ": ";
+      const int32_t c0 = skel->face_cell[2 * f], c1 = skel->face_cell[2 * f + 1];
+      const int lf0 = skel->face_lface[2 * f], lf1 = skel->face_lface[2 * f + 1];
+      if (c0 < 0 || c0 >= c->ncells || c1 < 0 || c1 >= c->ncells || lf0 < 0 || lf0 > 5 || lf1 < 0 || lf1 > 5) {
+        c->err = where + "cell or local face out of range";
+        return -1;
+      }
+      if (c0 == c1) {
+        c->err = where + "a cell glued to itself (periodic direction one cell thick) is not supported";
+        return -1;
+      }
+      if (!std::isfinite(skel->mu[f])) {
+        c->err = where + "mu is not finite";
+        return -1;
+      }
+      int seen = 0;
+      for (int q = 0; q < mhd::NQF; ++q) {
+        const int p = skel->face_perm[(int64_t)9 * f + q];
+        if (p < 0 || p >= mhd::NQF || (seen >> p & 1)) {
+          c->err = where + "face_perm is not a permutation of 0..8";
+          return -1;
+        }
+        seen |= 1 << p;
+      }
+      const int32_t *d0 = spaces->cell_dofs + (int64_t)c0 * mhd::NDOF;
+      const int32_t *d1 = spaces->cell_dofs + (int64_t)c1 * mhd::NDOF;
+      int32_t *sd = sdofs.data() + (int64_t)f * mhd::SKEL_DOFS;
+      for (int a = 0; a < 27; ++a) {
+        for (int b = 0; b < a; ++b)
+          if (d0[b] == d0[a]) {
+            c->err = where + "a cell holds a velocity node twice";
+            return -1;
+          }
+        for (int comp = 0; comp < 3; ++comp) sd[comp * mhd::SKEL_SLOTS + a] = d0[comp * 27 + a];
+      }
+      // node a = i + 3j + 9k lies on local face lf (z-, z+, y-, y+, x-, x+) when
+      // its coordinate along the face axis is 0 (lower side) or 2 (upper side)
+      auto on_face = [](int a, int lf) {
+        const int axis = 2 - lf / 2, coord = axis == 0 ? a % 3 : (axis == 1 ? (a / 3) % 3 : a / 9);
+        return coord == ((lf & 1) ? 2 : 0);
+      };
+      int matched = 0;
+      for (int a1 = 0; a1 < 27; ++a1) {
+        int slot = -1;
+        for (int a0 = 0; a0 < 27 && slot < 0; ++a0)
+          if (d0[a0] == d1[a1]) slot = a0;
+        if (slot >= 0) {
+          if (on_face(a1, lf1) && on_face(slot, lf0)) ++matched;
+        } else {
+          slot = 27 + a1;
+          for (int comp = 0; comp < 3; ++comp) sd[comp * mhd::SKEL_SLOTS + slot] = d1[comp * 27 + a1];
+        }
+        mslot[(size_t)f * 27 + a1] = (uint8_t)slot;
+      }
+      if (matched != mhd::NQF) {
+        c->err = where + "the two cells do not share the nodes of the given local faces";
+        return -1;
+      }
+    }
+  }
   c->block_style = spaces->block_style;
   c->h_dofs.assign(spaces->cell_dofs, spaces->cell_dofs + ntab);
   c->h_xyz.assign(mesh->cell_xyz, mesh->cell_xyz + (size_t)c->ncells * 24);
@@ -141,7 +218,8 @@ int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, co
     if (rc) return rc;
   }
   // sparsity pattern (host) and cell -> nonzero rank map (device)
-  mhd::build_pattern(c->ncells, spaces->cell_dofs, n, c->h_rowptr, c->h_colind);
+  mhd::build_pattern(c->ncells, spaces->cell_dofs, n, c->h_rowptr, c->h_colind, nskel,
+                     nskel > 0 ? sdofs.data() : nullptr);
   c->nnz = c->h_rowptr[n];
   if ((rc = dev_upload(c, &c->d_rowptr, c->h_rowptr.data(), (size_t)n + 1))) return rc;
   if ((rc = dev_upload(c, &c->d_colind, c->h_colind.data(), (size_t)c->nnz))) return rc;
@@ -224,9 +302,45 @@ int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, co
   MHD_HIP(hipMalloc(&c->d_val, sizeof(double) * std::max<int64_t>(c->nnz, 1)));
   MHD_HIP(hipMalloc(&c->d_x, sizeof(double) * std::max<int64_t>(n, 1)));
   MHD_HIP(hipMalloc(&c->d_r, sizeof(double) * std::max<int64_t>(n, 1)));
+  if (nskel > 0) {
+    // skeleton entries of every row, in increasing face order
+    std::vector<int32_t> srow(n + 1, 0);
+    for (int32_t g : sdofs)
+      if (g >= 0) srow[g + 1]++;
+    for (int64_t i = 0; i < n; ++i) srow[i + 1] += srow[i];
+    std::vector<int32_t> sentry(std::max<int32_t>(srow[n], 1)), fill(srow.begin(), srow.end() - 1);
+    for (int64_t k = 0; k < (int64_t)sdofs.size(); ++k)
+      if (sdofs[k] >= 0) sentry[fill[sdofs[k]]++] = (int32_t)k;
+    c->nskel = nskel;
+    if ((rc = dev_upload(c, &c->d_skel_dofs, sdofs.data(), sdofs.size()))) return rc;
+    if ((rc = dev_upload(c, &c->d_srow, srow.data(), srow.size()))) return rc;
+    if ((rc = dev_upload(c, &c->d_sentry, sentry.data(), sentry.size()))) return rc;
+    int32_t *d_fc = nullptr;
+    int8_t *d_lf = nullptr, *d_perm = nullptr;
+    uint8_t *d_ms = nullptr;
+    double *d_mu = nullptr;
+    rc = dev_upload(c, &d_fc, skel->face_cell, (size_t)2 * nskel);
+    if (!rc) rc = dev_upload(c, &d_lf, skel->face_lface, (size_t)2 * nskel);
+    if (!rc) rc = dev_upload(c, &d_perm, skel->face_perm, (size_t)9 * nskel);
+    if (!rc) rc = dev_upload(c, &d_ms, mslot.data(), mslot.size());
+    if (!rc) rc = dev_upload(c, &d_mu, skel->mu, (size_t)nskel);
+    if (!rc) rc = mhd_build_skeleton(c, d_fc, d_lf, d_perm, d_ms, d_mu);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) {
+      c->err = "skeleton face matrices: stream synchronisation failed";
+      rc = -2;
+    }
+    for (void *p : {(void *)d_fc, (void *)d_lf, (void *)d_perm, (void *)d_ms, (void *)d_mu})
+      if (p) (void)hipFree(p);
+    if (rc) return rc;
+  }
   // device address of this GPU's allocation-free residual workspace (looked up
   // once per device, with the context's device current)
   return mhd_noalloc_init(c);
+}
+
+int mhd_create(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces *spaces, const mhd_params *params,
+               int32_t device) {
+  return mhd_create_stabilized(out, mesh, spaces, params, nullptr, device);
 }
 
 void mhd_destroy(mhd_ctx *c) {
@@ -237,7 +351,8 @@ void mhd_destroy(mhd_ctx *c) {
   mhd_coarse_free(c);
   void *ptrs[] = {c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw,
                   c->d_rowptr, c->d_colind, c->d_rank, c->d_val, c->d_tperm, c->d_ref, c->d_fref, c->d_x, c->d_r,
-                  c->d_color_cells, c->d_crow, c->d_centry, c->d_ebuf, c->d_rbuf};
+                  c->d_color_cells, c->d_crow, c->d_centry, c->d_ebuf, c->d_rbuf, c->d_skel_dofs, c->d_skel_K,
+                  c->d_srow, c->d_sentry};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);

@@ -1029,6 +1029,160 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t *__restri
   if (row < n && lane == 0) y[row] = acc;
 }
 
+// Skeleton stabilisation (src/weakforms.jl:685-688):
+//   a_Lambda(u, v) = int_Lambda (1/2) mu h^2 [grad u] : [grad v],  h^2 = |face|.
+// [grad u] : [grad v] = sum_c sum_i [d_i u_c][d_i v_c]: the three velocity
+// components share one scalar face matrix over the face's 54 node slots,
+//   K_f[a][b] = (mu/2) |f| sum_q W_q G_q[a] . G_q[b],
+// G_q[a] the jump of the physical gradient of node a's Q2 function at face
+// point q (first cell's minus the second cell's, at the coinciding points),
+// W_q = w_q |det J| |J^-T n^| on the first cell.  a_Lambda is linear and
+// depends on the geometry only, so K_f is computed once per context.
+__global__ __launch_bounds__(64) void k_skel_matrix(int nf, const double *__restrict__ xyz,
+                                                    const int32_t *__restrict__ fcell,
+                                                    const int8_t *__restrict__ flf, const int8_t *__restrict__ fperm,
+                                                    const uint8_t *__restrict__ mslot, const double *__restrict__ mu,
+                                                    const FaceTables *__restrict__ F, double *__restrict__ K) {
+  __shared__ double X[2][24];
+  __shared__ double Jis[2][NQF][9];      // (J^-1)[l][i] of each side at the first side's point q
+  __shared__ double W[NQF];
+  __shared__ double G[NQF][SKEL_SLOTS][3];
+  const int tid = threadIdx.x;
+  for (int f = blockIdx.x; f < nf; f += gridDim.x) {
+    const int c0 = fcell[2 * f], c1 = fcell[2 * f + 1];
+    const int lf0 = flf[2 * f], lf1 = flf[2 * f + 1];
+    if (tid < 48) X[tid / 24][tid % 24] = xyz[(int64_t)(tid < 24 ? c0 : c1) * 24 + tid % 24];
+    for (int e = tid; e < NQF * SKEL_SLOTS * 3; e += 64) (&G[0][0][0])[e] = 0.0;
+    __syncthreads();
+    if (tid < 2 * NQF) {
+      const int s = tid / NQF, q = tid % NQF;
+      const int lf = s ? lf1 : lf0, qs = s ? fperm[(int64_t)NQF * f + q] : q;
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int v = 0; v < 8; ++v)
+        for (int i = 0; i < 3; ++i)
+          for (int l = 0; l < 3; ++l) J[i][l] += X[s][3 * v + i] * F->dq1[lf][qs][v][l];
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      const double id = 1.0 / det;
+      double *Ji = Jis[s][q];
+      Ji[0] = c00 * id;
+      Ji[3] = c01 * id;
+      Ji[6] = c02 * id;
+      Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+      Ji[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+      Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+      Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+      Ji[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+      Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+      if (s == 0) {
+        const int d = 2 - lf / 2;
+        const double nn = sqrt(Ji[3 * d] * Ji[3 * d] + Ji[3 * d + 1] * Ji[3 * d + 1] + Ji[3 * d + 2] * Ji[3 * d + 2]);
+        W[q] = F->w[q] * fabs(det) * nn;
+      }
+    }
+    __syncthreads();
+    // first cell: slots 0..26 (distinct); then the second cell subtracts its
+    // gradients at the slots mslot (distinct among themselves)
+    for (int s = 0; s < 2; ++s) {
+      for (int e = tid; e < NQF * 27; e += 64) {
+        const int q = e / 27, a = e % 27;
+        const int lf = s ? lf1 : lf0, qs = s ? fperm[(int64_t)NQF * f + q] : q;
+        const int slot = s ? mslot[(int64_t)27 * f + a] : a;
+        const double *g = F->dq2[lf][qs][a];
+        const double *Ji = Jis[s][q];
+        const double sg = s ? -1.0 : 1.0;
+        for (int i = 0; i < 3; ++i) G[q][slot][i] += sg * (Ji[i] * g[0] + Ji[3 + i] * g[1] + Ji[6 + i] * g[2]);
+      }
+      __syncthreads();
+    }
+    double area = 0.0;
+    for (int q = 0; q < NQF; ++q) area += W[q];
+    const double coef = 0.5 * mu[f] * area;
+    double *Kf = K + (int64_t)f * SKEL_SLOTS * SKEL_SLOTS;
+    for (int e = tid; e < SKEL_SLOTS * SKEL_SLOTS; e += 64) {
+      const int a = e / SKEL_SLOTS, b = e % SKEL_SLOTS;
+      double acc = 0.0;
+      for (int q = 0; q < NQF; ++q)
+        acc += W[q] * (G[q][a][0] * G[q][b][0] + G[q][a][1] * G[q][b][1] + G[q][a][2] * G[q][b][2]);
+      Kf[e] = coef * acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Skeleton rows of the residual: r_i += sum over the row's (face, component,
+// slot) entries of K_f[slot][:] . x at the face's slot dofs of that component
+// (Dirichlet values for non-free slots).  One thread per row, entries in
+// increasing face order: the sums do not depend on the launch.
+__global__ void k_skel_residual(int64_t n, const int32_t *__restrict__ srow, const int32_t *__restrict__ sentry,
+                                const int32_t *__restrict__ sdofs, const double *__restrict__ K,
+                                const double *__restrict__ x, const double *__restrict__ dir, double *__restrict__ r) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int p0 = srow[i], p1 = srow[i + 1];
+    if (p0 == p1) continue;
+    double acc = 0.0;
+    for (int p = p0; p < p1; ++p) {
+      const int32_t e = sentry[p];
+      const int f = e / SKEL_DOFS, cs = e % SKEL_DOFS;
+      const int comp = cs / SKEL_SLOTS, a = cs % SKEL_SLOTS;
+      const int32_t *sd = sdofs + (int64_t)f * SKEL_DOFS + comp * SKEL_SLOTS;
+      const double *Kr = K + ((int64_t)f * SKEL_SLOTS + a) * SKEL_SLOTS;
+      for (int b = 0; b < SKEL_SLOTS; ++b) {
+        const int32_t g = sd[b];
+        if (g == SKEL_NONE) continue;
+        acc += Kr[b] * (g >= 0 ? x[g] : dir[-(int64_t)g - 1]);
+      }
+    }
+    r[i] += acc;
+  }
+}
+
+// Skeleton rows of the Jacobian, added to the assembled CSR values: one
+// 64-lane wave per row; the row is accumulated in LDS (lane b adds K_f[slot][b]
+// at the rank of slot b's dof, found by binary search in the sorted row --
+// the slot dofs of one entry are distinct, so lanes never collide), entries
+// in increasing face order, then added to the row once.
+__global__ __launch_bounds__(64) void k_skel_jacobian(int64_t n, const int64_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ colind,
+                                                      const int32_t *__restrict__ srow,
+                                                      const int32_t *__restrict__ sentry,
+                                                      const int32_t *__restrict__ sdofs,
+                                                      const double *__restrict__ K, double *__restrict__ val) {
+  extern __shared__ double sk_lds[];
+  const int lane = threadIdx.x;
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const int p0 = srow[i], p1 = srow[i + 1];
+    if (p0 == p1) continue;
+    const int64_t rb = rowptr[i];
+    const int len = (int)(rowptr[i + 1] - rb);
+    for (int t = lane; t < len; t += 64) sk_lds[t] = 0.0;
+    __builtin_amdgcn_wave_barrier();
+    for (int p = p0; p < p1; ++p) {
+      const int32_t e = sentry[p];
+      const int f = e / SKEL_DOFS, cs = e % SKEL_DOFS;
+      const int comp = cs / SKEL_SLOTS, a = cs % SKEL_SLOTS;
+      if (lane < SKEL_SLOTS) {
+        const int32_t g = sdofs[(int64_t)f * SKEL_DOFS + comp * SKEL_SLOTS + lane];
+        if (g >= 0) {
+          int lo = 0, hi = len;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (colind[rb + mid] < g) lo = mid + 1;
+            else hi = mid;
+          }
+          if (lo < len && colind[rb + lo] == g)
+            sk_lds[lo] += K[((int64_t)f * SKEL_SLOTS + a) * SKEL_SLOTS + lane];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (int t = lane; t < len; t += 64) val[rb + t] += sk_lds[t];
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 }  // namespace
 
 int mhd_launch_jacobian(mhd_ctx *c, const double *d_x, double *d_val, hipStream_t s) {
@@ -1074,6 +1228,40 @@ int mhd_launch_jacobian(mhd_ctx *c, const double *d_x, double *d_val, hipStream_
                        d_val);
     MHD_HIP(hipGetLastError());
   }
+  return mhd_launch_skeleton_jacobian(c, d_val, s);
+}
+
+int mhd_build_skeleton(mhd_ctx *c, const int32_t *d_fcell, const int8_t *d_flf, const int8_t *d_fperm,
+                       const uint8_t *d_mslot, const double *d_mu) {
+  if (c->nskel <= 0) return 0;
+  MHD_HIP(hipMalloc(&c->d_skel_K, sizeof(double) * (size_t)c->nskel * SKEL_SLOTS * SKEL_SLOTS));
+  const int blocks = std::min(c->nskel, 1 << 20);
+  hipLaunchKernelGGL(k_skel_matrix, dim3(blocks), dim3(64), 0, c->stream, c->nskel, c->d_xyz, d_fcell, d_flf,
+                     d_fperm, d_mslot, d_mu, c->d_fref, c->d_skel_K);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+int mhd_launch_skeleton_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s) {
+  if (c->nskel <= 0 || c->n <= 0) return 0;
+  const int64_t blocks = std::min<int64_t>((c->n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_skel_residual, dim3((unsigned)blocks), dim3(256), 0, s, c->n, c->d_srow, c->d_sentry,
+                     c->d_skel_dofs, c->d_skel_K, d_x, c->d_dir, d_r);
+  MHD_HIP(hipGetLastError());
+  return 0;
+}
+
+int mhd_launch_skeleton_jacobian(mhd_ctx *c, double *d_val, hipStream_t s) {
+  if (c->nskel <= 0 || c->n <= 0) return 0;
+  const size_t lds = sizeof(double) * (size_t)c->max_row;
+  if (lds > 65536) {
+    c->err = "skeleton Jacobian: a CSR row longer than 8192 entries";
+    return -1;
+  }
+  const int64_t blocks = std::min<int64_t>(c->n, 1 << 20);
+  hipLaunchKernelGGL(k_skel_jacobian, dim3((unsigned)blocks), dim3(64), lds, s, c->n, c->d_rowptr, c->d_colind,
+                     c->d_srow, c->d_sentry, c->d_skel_dofs, c->d_skel_K, d_val);
+  MHD_HIP(hipGetLastError());
   return 0;
 }
 
@@ -1093,7 +1281,8 @@ int mhd_launch_residual_mode(mhd_ctx *c, const double *d_x, double *d_r, hipStre
                        c->d_dir, d_x, c->d_fref, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw, d_r, mode);
     MHD_HIP(hipGetLastError());
   }
-  return 0;
+  // the skeleton term depends on neither gamma nor cw (modes 1, 2: dR/dtheta)
+  return mode == 0 ? mhd_launch_skeleton_residual(c, d_x, d_r, s) : 0;
 }
 
 int mhd_launch_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s) {
@@ -1164,7 +1353,8 @@ int mhd_launch_residual_noalloc(mhd_ctx *c, const double *d_x, double *d_r, hipS
                        c->d_dir, d_x, c->d_fref, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw, d_r, 0);
     MHD_HIP(hipGetLastError());
   }
-  return 0;
+  // the skeleton face matrices are context memory made at creation
+  return mhd_launch_skeleton_residual(c, d_x, d_r, s);
 }
 
 // val_t[p] = val[tperm[p]]: the CSR values of A^T on the (symmetric) pattern

@@ -45,14 +45,25 @@ struct FaceTables {
   double dq1[6][NQF][8][3];
   double rt[6][NQF][36][3];
   double drt[6][NQF][36][3][3];  // d rt_l / d xi_m  ([l][m])
+  double dq2[6][NQF][27][3];     // Q2 reference gradients (skeleton stabilisation)
 };
+
+// Skeleton stabilisation: the velocity nodes of a face are the 27 of its first
+// cell (slots 0..26) and those of the second cell not shared with the first
+// (slot 27 + local node); SKEL_NONE marks the unused slots.
+constexpr int SKEL_SLOTS = 54;
+constexpr int SKEL_DOFS = 3 * SKEL_SLOTS;   // [component][slot]
+constexpr int32_t SKEL_NONE = INT32_MIN;
 
 void build_ref_tables(RefTables *T);
 void build_face_tables(FaceTables *F);
 
 // Pattern (host): CSR with int64 row pointers and int32 sorted column indices.
+// skel_dofs ([nskel][SKEL_DOFS], may be null) adds the velocity couplings of
+// the skeleton faces: every slot dof of a component with every other of it.
 void build_pattern(int32_t ncells, const int32_t *cell_dofs, int64_t n,
-                   std::vector<int64_t> &rowptr, std::vector<int32_t> &colind);
+                   std::vector<int64_t> &rowptr, std::vector<int32_t> &colind,
+                   int32_t nskel = 0, const int32_t *skel_dofs = nullptr);
 
 }  // namespace mhd
 
@@ -114,6 +125,12 @@ struct mhd_ctx {
   double *d_ebuf = nullptr;           // [ncells][NENT] element buffer (allocated on first use)
   double *d_rbuf = nullptr;           // [ncells][NDOF] element residuals (allocated on first use)
   double *d_scratch = nullptr;        // this device's allocation-free residual workspace (not owned)
+  // skeleton stabilisation (mhd_create_stabilized)
+  int32_t nskel = 0;
+  int32_t *d_skel_dofs = nullptr;     // [nskel][SKEL_DOFS] slot dofs (free id, -(k+1) Dirichlet, SKEL_NONE)
+  double *d_skel_K = nullptr;         // [nskel][54][54] face matrices of one velocity component
+  int32_t *d_srow = nullptr;          // [n + 1] skeleton entries of each row
+  int32_t *d_sentry = nullptr;        // face * SKEL_DOFS + component * 54 + slot, increasing face order per row
   // solver workspace (allocated lazily)
   struct Solver *solver = nullptr;
   // coarser level of the [u,j] block solve (owned), see mhd_attach_coarse
@@ -166,6 +183,10 @@ int mhd_launch_residual_mode(mhd_ctx *c, const double *d_x, double *d_r, hipStre
 int mhd_launch_transpose_values(mhd_ctx *c, const double *d_val, double *d_val_t, hipStream_t s);
 int mhd_noalloc_init(mhd_ctx *c);
 int mhd_launch_residual_noalloc(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s);
+int mhd_build_skeleton(mhd_ctx *c, const int32_t *d_fcell, const int8_t *d_flf, const int8_t *d_fperm,
+                       const uint8_t *d_mslot, const double *d_mu);
+int mhd_launch_skeleton_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s);
+int mhd_launch_skeleton_jacobian(mhd_ctx *c, double *d_val, hipStream_t s);
 
 #define MHD_HIP(call)                                                                 \
   do {                                                                                \

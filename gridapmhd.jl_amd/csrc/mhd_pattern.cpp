@@ -4,7 +4,9 @@
 // every field block the weak form touches (src/weakforms.jl:311: uu, up, uj,
 // pu, jj, jphi, ju, phij); Dirichlet rows and columns are dropped.  The
 // pattern is structurally symmetric, so one CSR (sorted columns) serves as
-// CSR of A and, read column-wise, as Gridap's CSC of A.
+// CSR of A and, read column-wise, as Gridap's CSC of A.  The skeleton
+// stabilisation adds, per interior face, the same-component velocity couplings
+// of the face's two cells (the jump(grad u) : jump(grad v) face integrand).
 #include <algorithm>
 #include <atomic>
 #include <thread>
@@ -42,7 +44,7 @@ void parallel_chunks(int64_t n, int64_t chunk, F &&f) {
 }  // namespace
 
 void build_pattern(int32_t ncells, const int32_t *cell_dofs, int64_t n, std::vector<int64_t> &rowptr,
-                   std::vector<int32_t> &colind) {
+                   std::vector<int32_t> &colind, int32_t nskel, const int32_t *skel_dofs) {
   // dof -> (cell, local) incidence
   std::vector<int64_t> inc_ptr(n + 1, 0);
   for (int64_t c = 0; c < ncells; ++c)
@@ -65,6 +67,18 @@ void build_pattern(int32_t ncells, const int32_t *cell_dofs, int64_t n, std::vec
         }
       }
   }
+  // dof -> (skeleton face, component slot) incidence
+  std::vector<int64_t> sinc_ptr(skel_dofs ? n + 1 : 1, 0);
+  std::vector<int32_t> sinc;
+  if (skel_dofs) {
+    for (int64_t k = 0; k < (int64_t)nskel * SKEL_DOFS; ++k)
+      if (skel_dofs[k] >= 0) sinc_ptr[skel_dofs[k] + 1]++;
+    for (int64_t i = 0; i < n; ++i) sinc_ptr[i + 1] += sinc_ptr[i];
+    sinc.resize(sinc_ptr[n]);
+    std::vector<int64_t> fill(sinc_ptr.begin(), sinc_ptr.end() - 1);
+    for (int64_t k = 0; k < (int64_t)nskel * SKEL_DOFS; ++k)
+      if (skel_dofs[k] >= 0) sinc[fill[skel_dofs[k]]++] = (int32_t)(k / SKEL_SLOTS);  // face * 3 + component
+  }
   unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
   std::vector<std::vector<int32_t>> marks(nt);
   std::vector<std::vector<int32_t>> bufs(nt);
@@ -84,6 +98,16 @@ void build_pattern(int32_t ncells, const int32_t *cell_dofs, int64_t n, std::vec
         out.push_back(g);
       }
     }
+    if (skel_dofs)
+      for (int64_t p = sinc_ptr[i]; p < sinc_ptr[i + 1]; ++p) {
+        const int32_t *sd = skel_dofs + (int64_t)sinc[p] * SKEL_SLOTS;
+        for (int v = 0; v < SKEL_SLOTS; ++v) {
+          const int32_t g = sd[v];
+          if (g < 0 || mark[g] == (int32_t)i) continue;
+          mark[g] = (int32_t)i;
+          out.push_back(g);
+        }
+      }
     std::sort(out.begin(), out.end());
   };
   rowptr.assign(n + 1, 0);

@@ -146,6 +146,7 @@ void build_face_tables(FaceTables *F) {
       std::memcpy(F->dq1[f][q], P.dq1, sizeof(P.dq1));
       std::memcpy(F->rt[f][q], P.rt, sizeof(P.rt));
       std::memcpy(F->drt[f][q], P.drt, sizeof(P.drt));
+      std::memcpy(F->dq2[f][q], P.dq2, sizeof(P.dq2));
     }
   }
 }

@@ -434,6 +434,57 @@ def _setup_hex_spaces(mesh, u_tags, j_tags, thin_walls, style) -> "Spaces":
     return sp
 
 
+def skeleton_faces(sp: Spaces) -> dict:
+    """Interior faces of the mesh, the skeleton of the skeleton stabilisation
+    (``Skeleton(Triangulation(model, tags=fluid))``, src/geometry.jl:138-140;
+    every cell is fluid here).  A face shared by two cells, periodic images
+    included, joins its first appearance (cell order, local face order) with
+    the second: ``cell`` / ``lface`` (nf, 2), and ``perm`` (nf, 9) the 3x3 face
+    Gauss point (lower tangential axis fastest) of the second cell at each point
+    of the first.  The points are matched through the face's corner vertices
+    (topological ids, so periodic faces match too)."""
+    cached = sp.__dict__.get("_skeleton")
+    if cached is not None:
+        return cached
+    fid = sp.face_ids.ravel()
+    order = np.argsort(fid, kind="stable")               # (cell, local face) pairs by face, first appearance first
+    f_sorted = fid[order]
+    starts = np.nonzero(np.r_[True, f_sorted[1:] != f_sorted[:-1]])[0]
+    counts = np.diff(np.r_[starts, f_sorted.size])
+    if (counts > 2).any():
+        raise ValueError("a face is shared by more than two cells")
+    two = starts[counts == 2]
+    first, second = order[two], order[two + 1]
+    keep = np.argsort(first, kind="stable")
+    first, second = first[keep], second[keep]
+    cell = np.stack([first // 6, second // 6], 1)
+    lface = np.stack([first % 6, second % 6], 1)
+    # corners (i0, i1) of each side in face coordinates: local vertex side<<d | i0<<ta | i1<<tb
+    corner_v = np.zeros((6, 4), dtype=np.int64)
+    for f in range(6):
+        d, (ta, tb) = FACE_AXIS[f], _tangential_axes(FACE_AXIS[f])
+        for k in range(4):
+            corner_v[f, k] = (FACE_SIDE[f] << d) | ((k & 1) << ta) | ((k >> 1) << tb)
+    verts = sp.cell_verts
+    P = np.take_along_axis(verts[cell[:, 0]], corner_v[lface[:, 0]], 1)      # (nf, 4)
+    M = np.take_along_axis(verts[cell[:, 1]], corner_v[lface[:, 1]], 1)
+    match = P[:, :, None] == M[:, None, :]                                   # plus corner k -> minus corner m
+    if not (match.sum(2) == 1).all():
+        raise ValueError("the two cells of an interior face do not share its four corners")
+    m = np.argmax(match, 2)
+    A, B, C = m[:, 0], m[:, 1], m[:, 2]                 # minus corners of plus (0,0), (1,0), (0,1)
+    bit = lambda v, k: (v >> k) & 1
+    qa = np.arange(9) % 3
+    qb = np.arange(9) // 3
+    s0 = 2 * bit(A, 0)[:, None] + (bit(B, 0) - bit(A, 0))[:, None] * qa + (bit(C, 0) - bit(A, 0))[:, None] * qb
+    s1 = 2 * bit(A, 1)[:, None] + (bit(B, 1) - bit(A, 1))[:, None] * qa + (bit(C, 1) - bit(A, 1))[:, None] * qb
+    perm = s0 + 3 * s1
+    assert (np.sort(perm, 1) == np.arange(9)).all()
+    out = dict(cell=cell.astype(np.int32), lface=lface.astype(np.int8), perm=perm.astype(np.int8))
+    sp.__dict__["_skeleton"] = out
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Geometry helpers (Q1 map) and interpolation
 

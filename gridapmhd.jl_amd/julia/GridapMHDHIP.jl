@@ -261,6 +261,10 @@ are those of Gridap 0.17 / 0.18; the face-orientation signs of the RT space
 are read from `get_sign_flip`, which assumes Gridap's own orientation rule.
 """
 function extract_problem(U, params)
+  # the skeleton stabilisation needs the interior-face list of mhd_create_stabilized,
+  # which this glue does not extract yet
+  isempty(get(params[:bcs], :stabilization, [])) ||
+    error("GridapMHDHIP: params[:bcs][:stabilization] is not passed to libmhd by this glue yet")
   model = params[:model]
   Ω = Triangulation(model)
   ncells = num_cells(Ω)

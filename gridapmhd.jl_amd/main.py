@@ -17,7 +17,7 @@ import numpy as np
 
 from . import _lib, fespaces
 from .meshers import CartesianMesh
-from .parameters import add_default_params, fluid_physics
+from .parameters import add_default_params, physics
 
 PRECISION = {"fp64": 0, "mixed": 1, "mixed_bf16": 2, "mixed_fp16": 3}
 
@@ -86,7 +86,7 @@ def _fe_operator(sp, params) -> _lib.Context:
     """``_fe_operator`` (src/main.jl:207-233): the operator lives in libmhd,
     with the [u,j] V-cycle hierarchy of the solver parameters attached."""
     s = params["solver"]
-    return _lib.context_with_levels(sp, fluid_physics(params["fluid"]), levels=s.get("levels", 1),
+    return _lib.context_with_levels(sp, physics(params), levels=s.get("levels", 1),
                                     coarse_iters=int(s.get("coarse_iters", 0)),
                                     smoother_weight=float(s.get("smoother_weight", 0.175)),
                                     coarse_step=s.get("coarse_step", "minres"))
@@ -111,7 +111,7 @@ def initial_guess(x0, sp, ctx, params):
     if isinstance(x0, str) and x0 == "solve":
         if params["fluid"]["convection"] == "none":
             raise ValueError("Convection must be enabled to use initial guess :solve")
-        phys = fluid_physics(params["fluid"])
+        phys = physics(params)
         ctx.set_params(dict(phys, convection="none"))
         x, _, _ = ctx.newton_solve(np.zeros(sp.ndofs), _solver_opts(ctx, params))
         ctx.set_params(phys)

@@ -162,7 +162,13 @@ def params_bcs(params):
     if bcs["t"] != []:
         raise NotImplementedError("Boundary tranction not yet implemented")  # as src/weakforms.jl:164-166
     if bcs["stabilization"] != []:
-        raise NotImplementedError("skeleton stabilisation is not part of the GPU path")
+        # src/parameters.jl:968-977: the domain defaults to the fluid's
+        fluid_dom = params["fluid"]["domain"]
+        bcs["stabilization"] = _check_and_add_weak(bcs["stabilization"], {"domain": False, "μ": True},
+                                                   {"domain": fluid_dom}, params, "[:bcs][:stabilization]")
+        for st in bcs["stabilization"]:
+            if st["domain"] != fluid_dom:
+                raise NotImplementedError("skeleton stabilisation on a domain other than the fluid's")
     if bcs["thin_wall"] != []:
         bcs["thin_wall"] = _check_and_add_weak(bcs["thin_wall"], {"domain": True, "cw": True, "τ": True, "jw": False},
                                                {"jw": 0.0}, params, "[:bcs][:thin_wall]")
@@ -208,3 +214,12 @@ def fluid_physics(fluid) -> Dict[str, Any]:
     return dict(alpha=float(fluid["α"]), beta=float(fluid["β"]), gamma=float(fluid["γ"]), sigma=float(fluid["σ"]),
                 B=np.asarray(fluid["B"], float), f0=f0, fx=fx, g0=g0, gx=gx, zeta_u=float(fluid["ζᵤ"]),
                 zeta_j=float(fluid["ζⱼ"]), convection=fluid["convection"])
+
+
+def physics(params) -> Dict[str, Any]:
+    """The flat parameter set of the C ABI for processed ``params``: the fluid
+    terms plus the skeleton stabilisation weight (the entries' mu summed: each
+    covers the whole fluid skeleton)."""
+    phys = fluid_physics(params["fluid"])
+    phys["stab_mu"] = float(sum(float(st["μ"]) for st in (params["bcs"].get("stabilization") or [])))
+    return phys

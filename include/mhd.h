@@ -105,11 +105,37 @@ typedef struct {
   int32_t krylov_per_step[64];
 } mhd_solve_info;
 
+/* Interior faces of the skeleton stabilisation
+ *   a_Lambda(u, v) = int_Lambda (1/2) mu h^2 [grad u] : [grad v]
+ * (src/weakforms.jl:685-688), Lambda the interior faces of the fluid domain,
+ * h = sqrt(face area) (the cell size of the skeleton triangulation,
+ * src/geometry.jl:150-158), mu per face (the sum over the
+ * params[:bcs][:stabilization] entries that hold it, src/weakforms.jl:168-177).
+ * Face k joins cell face_cell[2k] through its local face face_lface[2k] with
+ * cell face_cell[2k+1] through face_lface[2k+1]; face_perm[9k + q] is the 3x3
+ * face Gauss point (lower tangential axis fastest) of the second cell that
+ * coincides with point q of the first. */
+typedef struct {
+  int32_t nfaces;
+  int32_t pad_;
+  const int32_t *face_cell;     /* [nfaces][2] */
+  const int8_t *face_lface;     /* [nfaces][2] local faces 0..5 */
+  const int8_t *face_perm;      /* [nfaces][9] */
+  const double *mu;             /* [nfaces] */
+} mhd_skeleton;
+
 /* Create a context on `device`: copies inputs to HBM, builds the CSR/CSC
  * pattern and the cell-to-nonzero map.  Replaces _fe_operator +
  * SparseMatrixAssembler (src/main.jl:218-224). */
 int mhd_create(mhd_ctx **ctx, const mhd_mesh *mesh, const mhd_spaces *spaces,
                const mhd_params *params, int32_t device);
+/* mhd_create with the skeleton stabilisation in the residual and Jacobian
+ * (the params_Λ loop of src/weakforms.jl:227-229, 248-250).  The pattern gains
+ * the velocity couplings of the two cells of every face (same component);
+ * the face matrices (geometry only, a_Lambda is linear) are computed once
+ * here and kept in HBM (54 x 54 doubles per face). */
+int mhd_create_stabilized(mhd_ctx **ctx, const mhd_mesh *mesh, const mhd_spaces *spaces,
+                          const mhd_params *params, const mhd_skeleton *skeleton, int32_t device);
 void mhd_destroy(mhd_ctx *ctx);
 const char *mhd_last_error(const mhd_ctx *ctx);
 

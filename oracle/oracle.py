@@ -16,6 +16,10 @@ Layers restated here (reference files read as text):
     restated: it converges to the solution of the same discrete system, which
     the direct solve fixes.
   * Post-processing norms of ``_hunt`` (src/Applications/hunt.jl:247-260).
+  * Skeleton stabilisation a_Lambda (src/weakforms.jl:685-688) in numpy
+    (``skeleton_system``): interior faces found from the cells' shared
+    face-centre velocity nodes, face points matched by physical position (up
+    to the periodic translation), surface measure |dx/ds x dx/dt|.
 
 oracle/cpu_baseline.py + cpu_step.c price one Newton step on the host CPU
 (timing only, for the benchmark's CPU baseline).
@@ -136,6 +140,16 @@ class Oracle:
         L.orc_point_values.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         assert L.orc_struct_size() == ctypes.sizeof(OrcProblem)
         self._pattern = None
+        self.stab_mu = float(problem.get("stab_mu", 0.0))
+        self._skel = None
+
+    def skeleton(self):
+        """(K (nf, 54, 54), dofs (nf, 3, 54)) of the skeleton stabilisation, or None."""
+        if self.stab_mu == 0.0:
+            return None
+        if self._skel is None:
+            self._skel = skeleton_system(self.cell_xyz, self.cell_dofs, self.stab_mu)
+        return self._skel
 
     # ---- element level ----------------------------------------------------
     def cell_residuals(self, x, c0=0, c1=None):
@@ -181,6 +195,13 @@ class Oracle:
             fd = cd[self.keep_face_cell(), OFF_J:OFF_J + 36]
             m = fd >= 0
             np.add.at(r, fd[m], rf[m])
+        sk = self.skeleton()
+        if sk is not None:
+            K, dofs = sk
+            xv = np.where(dofs >= 0, np.asarray(x)[np.maximum(dofs, 0)], self.dir_vals[np.maximum(-dofs - 1, 0)])
+            rs = np.einsum("fab,fcb->fca", K, xv)
+            m = dofs >= 0
+            np.add.at(r, dofs[m], rs[m])
         return r
 
     def keep_face_cell(self):
@@ -207,6 +228,15 @@ class Oracle:
             m = (R >= 0) & (C >= 0)
             rows.append(R[m]); cols.append(C[m])
             vals.append(Kf[m] if values else np.zeros(int(m.sum())))
+        sk = self.skeleton()
+        if sk is not None:
+            K, dofs = sk
+            nf = K.shape[0]
+            R = np.broadcast_to(dofs[:, :, :, None], (nf, 3, 54, 54))
+            C = np.broadcast_to(dofs[:, :, None, :], (nf, 3, 54, 54))
+            m = (R >= 0) & (C >= 0)
+            rows.append(R[m]); cols.append(C[m])
+            vals.append(np.broadcast_to(K[:, None], (nf, 3, 54, 54))[m] if values else np.zeros(int(m.sum())))
         return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
 
     def jacobian(self, x) -> sps.csc_matrix:
@@ -252,6 +282,115 @@ class Oracle:
         out = np.zeros((cells.size, 17))
         lib().orc_point_values(ctypes.byref(self.P), _ptr(x), cells.size, _ptr(cells), _ptr(xi), _ptr(out))
         return out
+
+
+# ---------------------------------------------------------------------------
+# Skeleton stabilisation  a_Lambda(u, v) = int_Lambda (1/2) mu h^2 [grad u] : [grad v]
+# (src/weakforms.jl:685-688), Lambda the interior faces, h = |face|^(1/2)
+# (get_cell_size of the skeleton triangulation: measure^(1/dim) with dim 2,
+# src/geometry.jl:150-158).
+
+def _gauss3():
+    x, w = np.polynomial.legendre.leggauss(3)
+    return (x + 1.0) / 2.0, w / 2.0
+
+
+def _lagrange2(t):
+    """Quadratic Lagrange basis on the nodes 0, 1/2, 1 and its derivative."""
+    L = np.stack([2.0 * (t - 0.5) * (t - 1.0), -4.0 * t * (t - 1.0), 2.0 * t * (t - 0.5)], -1)
+    dL = np.stack([4.0 * t - 3.0, 4.0 - 8.0 * t, 4.0 * t - 1.0], -1)
+    return L, dL
+
+
+def q2_ref_gradients(xi):
+    """Reference gradients (n, 27, 3) of the Q2 node functions a = i + 3j + 9k."""
+    L, dL = zip(*[_lagrange2(xi[:, d]) for d in range(3)])
+    out = np.zeros((xi.shape[0], 27, 3))
+    for a in range(27):
+        i, j, k = a % 3, (a // 3) % 3, a // 9
+        out[:, a, 0] = dL[0][:, i] * L[1][:, j] * L[2][:, k]
+        out[:, a, 1] = L[0][:, i] * dL[1][:, j] * L[2][:, k]
+        out[:, a, 2] = L[0][:, i] * L[1][:, j] * dL[2][:, k]
+    return out
+
+
+def trilinear_map(X, xi):
+    """Points (n, 3) and Jacobians J[i][l] = dx_i / dxi_l (n, 3, 3) of cells X (n, 8, 3) at xi (n, 3)."""
+    n = xi.shape[0]
+    N = np.ones((n, 8))
+    dN = np.ones((n, 8, 3))
+    for v in range(8):
+        b = [(v >> d) & 1 for d in range(3)]
+        for d in range(3):
+            f = xi[:, d] if b[d] else 1.0 - xi[:, d]
+            df = 1.0 if b[d] else -1.0
+            N[:, v] *= f
+            for l in range(3):
+                dN[:, v, l] *= df if l == d else f
+    return np.einsum("nv,nvi->ni", N, X), np.einsum("nvi,nvl->nil", X, dN)
+
+
+def _face_ref_points(lf):
+    """3x3 Gauss points and weights on local face lf (z-, z+, y-, y+, x-, x+)."""
+    g, w = _gauss3()
+    d = 2 - lf // 2
+    t = [a for a in range(3) if a != d]
+    P = np.zeros((9, 3))
+    P[:, d] = float(lf & 1)
+    P[:, t[0]] = np.tile(g, 3)
+    P[:, t[1]] = np.repeat(g, 3)
+    return P, np.outer(w, w).ravel(), t
+
+
+def interior_faces(cell_dofs):
+    """(cell, local face) pairs (nf, 2, 2) of the faces two cells share: the
+    faces whose centre velocity node (local node with coordinate 0 or 2 along
+    the face axis, 1 along the others) carries the same dof in both cells."""
+    centre = []
+    for lf in range(6):
+        d = 2 - lf // 2
+        c = [1, 1, 1]
+        c[d] = 2 if lf & 1 else 0
+        centre.append(c[0] + 3 * c[1] + 9 * c[2])
+    key = cell_dofs[:, centre].astype(np.int64).ravel()
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    same = np.nonzero(ks[1:] == ks[:-1])[0]
+    a, b = order[same], order[same + 1]
+    return np.stack([np.stack([a // 6, a % 6], 1), np.stack([b // 6, b % 6], 1)], 1)
+
+
+def skeleton_system(cell_xyz, cell_dofs, mu):
+    """Face matrices K (nf, 54, 54) of one velocity component over the 27
+    velocity nodes of each side (first side, then second), and their dofs
+    (nf, 3, 54) per component; a node shared by both sides appears twice
+    (the assembly sums them)."""
+    pairs = interior_faces(cell_dofs)
+    nf = pairs.shape[0]
+    X = cell_xyz.reshape(-1, 8, 3)
+    K = np.zeros((nf, 54, 54))
+    dofs = np.zeros((nf, 3, 54), dtype=np.int64)
+    for f in range(nf):
+        (c0, l0), (c1, l1) = pairs[f]
+        P0, w, t = _face_ref_points(l0)
+        P1, _, _ = _face_ref_points(l1)
+        x0, J0 = trilinear_map(np.repeat(X[c0][None], 9, 0), P0)
+        x1, J1 = trilinear_map(np.repeat(X[c1][None], 9, 0), P1)
+        # the second side's points in the first side's order (translation: periodic images)
+        x1s = x1 + (x0.mean(0) - x1.mean(0))
+        dist = np.linalg.norm(x0[:, None] - x1s[None], axis=2)
+        m = dist.argmin(1)
+        scale = np.linalg.norm(x0[0] - x0[8])
+        assert (dist[np.arange(9), m] < 1e-9 * scale).all() and np.unique(m).size == 9, "face points do not match"
+        dA = w * np.linalg.norm(np.cross(J0[:, :, t[0]], J0[:, :, t[1]]), axis=1)
+        g0 = np.einsum("qli,qal->qai", np.linalg.inv(J0), q2_ref_gradients(P0))
+        g1 = np.einsum("qli,qal->qai", np.linalg.inv(J1[m]), q2_ref_gradients(P1[m]))
+        G = np.concatenate([g0, -g1], axis=1)                      # (9, 54, 3) jumps
+        K[f] = 0.5 * mu * dA.sum() * np.einsum("q,qai,qbi->ab", dA, G, G)
+        for b in range(3):
+            dofs[f, b, :27] = cell_dofs[c0, b * 27:(b + 1) * 27]
+            dofs[f, b, 27:] = cell_dofs[c1, b * 27:(b + 1) * 27]
+    return K, dofs
 
 
 # ---------------------------------------------------------------------------

@@ -170,7 +170,7 @@ def expansion_case(name):
     params = applications.expansion_params(mesh=EXPANSION_MSH, **EXPANSION_CASES[name])
     full = parameters.add_default_params(params)
     sp = m.setup_fe_spaces(full)
-    return params, sp, m.fluid_physics(full["fluid"])
+    return params, sp, parameters.physics(full)
 
 
 def load_fixture(name):

@@ -1,0 +1,113 @@
+"""The oracle's skeleton stabilisation a_Lambda (src/weakforms.jl:685-688),
+
+    a_Lambda(u, v) = int_Lambda (1/2) mu h^2 [grad u] : [grad v],   h = |face|^(1/2)
+
+(h: the cell size of the skeleton triangulation, measure^(1/dim) with dim 2,
+src/geometry.jl:150-158), pinned by closed forms on boxes of distinct,
+non-uniform sizes along every axis:
+
+* a kink |x - xm| across an interior plane: v^T K u = 2 mu sum_f |f|^2 over
+  the plane's faces (jump of d_x u = 2); fails for a wrong h (the cell volume
+  instead of the face area), a wrong 1/2 or a wrong normal-derivative scaling;
+* (x - xm)_+ y: jump y on the plane, v^T K u = (mu/2) sum_f |f| int_f y^2
+  (the tangential weighting of the face quadrature);
+* the same kink along z, and across components (zero: [grad u] : [grad v]
+  pairs equal components only);
+* continuous gradients have no jump: K u = 0 for a global quadratic on
+  affine cells, and for a linear field on jittered (non-affine) cells with a
+  periodic direction (the periodic faces are interior faces).
+
+The interior faces the oracle finds (cells sharing a face-centre velocity
+node) are the ones the product's face list names."""
+import numpy as np
+import pytest
+
+import cases
+
+MU = 0.7
+
+
+def _stretch(P):
+    out = P.copy()
+    out[:, 0] = 1.3 * P[:, 0] + 0.4 * P[:, 0] ** 2
+    out[:, 1] = 0.6 * P[:, 1] + 0.5 * P[:, 1] ** 3
+    out[:, 2] = 0.9 * P[:, 2] + 0.3 * P[:, 2] ** 2
+    return out
+
+
+def _spaces(nc, periodic=(False, False, False), jitter=0.0, stretch=True):
+    import gridapmhd_amd as g
+    mesh = g.meshers.cartesian_model((0.0, 1.0, 0.0, 1.0, 0.0, 1.0), nc, periodic, map=_stretch if stretch else None)
+    if jitter:
+        g.meshers.jitter_nodes(mesh, jitter, 7)
+    return mesh, g.fespaces.setup_fe_spaces(mesh, [], [], style="badia2024")
+
+
+def _skeleton_matrix(orc_mod, sp, mu=MU):
+    phys = cases.cfd_physics(10.0, 5.0, convection="none")
+    x = np.zeros(sp.ndofs)
+    A1 = orc_mod.Oracle(cases.problem_dict(sp, dict(phys, stab_mu=mu))).jacobian(x)
+    A0 = orc_mod.Oracle(cases.problem_dict(sp, phys)).jacobian(x)
+    return (A1 - A0).tocsr()
+
+
+def _u_vector(sp, fn):
+    import gridapmhd_amd as g
+    x, _ = g.fespaces.interpolate(sp, u=fn)
+    return x
+
+
+def test_interior_faces_match_product_list(orc_mod):
+    import gridapmhd_amd as g
+    for nc, per in (((3, 4, 5), (False, False, False)), ((3, 3, 4), (False, False, True))):
+        _, sp = _spaces(nc, per)
+        nx, ny, nz = nc
+        want = (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1) + (nx * ny if per[2] else 0)
+        pairs = orc_mod.interior_faces(sp.cell_dofs)
+        sk = g.fespaces.skeleton_faces(sp)
+        assert pairs.shape[0] == want == sk["cell"].shape[0]
+        a = {tuple(sorted(map(tuple, p))) for p in pairs.tolist()}
+        b = {tuple(sorted(((int(c0), int(l0)), (int(c1), int(l1)))))
+             for (c0, c1), (l0, l1) in zip(sk["cell"], sk["lface"])}
+        assert a == b
+
+
+def test_skeleton_closed_forms(orc_mod):
+    mesh, sp = _spaces((4, 3, 3))
+    K = _skeleton_matrix(orc_mod, sp)
+    xs, ys, zs = (np.unique(np.round(mesh.node_xyz[:, a], 14)) for a in range(3))
+    dx, dy, dz = np.diff(xs), np.diff(ys), np.diff(zs)
+    xm, zm = xs[2], zs[1]
+    zero = lambda X: 0.0 * X[:, 0]
+    kx = _u_vector(sp, lambda X: np.stack([np.abs(X[:, 0] - xm), zero(X), zero(X)], 1))
+    got = kx @ (K @ kx)
+    want = 2.0 * MU * np.sum(np.outer(dy, dz) ** 2)
+    assert abs(got - want) < 1e-11 * want, (got, want)
+    ky = _u_vector(sp, lambda X: np.stack([zero(X), np.maximum(X[:, 0] - xm, 0.0) * X[:, 1], zero(X)], 1))
+    got = ky @ (K @ ky)
+    want = 0.5 * MU * np.sum(np.outer(dy, dz) * np.outer((ys[1:] ** 3 - ys[:-1] ** 3) / 3.0, dz))
+    assert abs(got - want) < 1e-11 * want, (got, want)
+    kz = _u_vector(sp, lambda X: np.stack([zero(X), zero(X), np.abs(X[:, 2] - zm)], 1))
+    got = kz @ (K @ kz)
+    want = 2.0 * MU * np.sum(np.outer(dx, dy) ** 2)
+    assert abs(got - want) < 1e-11 * want, (got, want)
+    cross = _u_vector(sp, lambda X: np.stack([zero(X), np.abs(X[:, 0] - xm), zero(X)], 1))
+    assert abs(cross @ (K @ kx)) < 1e-13 * np.abs(K).max() * kx.size
+    # symmetric, positive semi-definite
+    assert abs(K - K.T).max() <= 1e-14 * abs(K).max()
+    assert kx @ (K @ kx) > 0 and ky @ (K @ ky) > 0
+    quad = _u_vector(sp, lambda X: np.stack([X[:, 0] ** 2 + X[:, 1] * X[:, 2], X[:, 0] * X[:, 1],
+                                             X[:, 2] ** 2 - X[:, 0]], 1))
+    assert np.abs(K @ quad).max() < 1e-12 * abs(K).max() * np.abs(quad).max()
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.25])
+def test_linear_fields_have_no_jump_with_periodic_faces(orc_mod, jitter):
+    _, sp = _spaces((3, 3, 4), (False, False, True), jitter=jitter, stretch=jitter == 0.0)
+    K = _skeleton_matrix(orc_mod, sp)
+    lin = _u_vector(sp, lambda X: np.stack([X[:, 0] + 2.0 * X[:, 1], 3.0 * X[:, 0] - X[:, 1], X[:, 0] + 0.5], 1))
+    assert np.abs(K @ lin).max() < 1e-12 * abs(K).max() * np.abs(lin).max()
+    # a field whose gradient jumps is not in the kernel
+    kink = _u_vector(sp, lambda X: np.stack([np.abs(X[:, 0] - 0.5 * (X[:, 0].min() + X[:, 0].max())), 0 * X[:, 0],
+                                             0 * X[:, 0]], 1))
+    assert kink @ (K @ kink) > 1e-3 * MU

@@ -55,6 +55,28 @@ def test_thin_wall_params():
     assert full["bcs"]["j"]["tags"] == []
 
 
+def test_stabilization_params():
+    """params[:bcs][:stabilization] (src/parameters.jl:950-977): mu mandatory,
+    the domain defaults to the fluid's; the flat physics carries the summed mu."""
+    p = _params()
+    p["bcs"]["stabilization"] = {"μ": 0.3}
+    full = g.add_default_params(p)
+    assert full["bcs"]["stabilization"] == [{"μ": 0.3, "domain": None}]
+    assert g.parameters.physics(full)["stab_mu"] == 0.3
+    p = _params()
+    p["bcs"]["stabilization"] = [{"μ": 0.3}, {"μ": 0.2, "domain": None}]
+    assert abs(g.parameters.physics(g.add_default_params(p))["stab_mu"] - 0.5) < 1e-15
+    assert g.parameters.physics(g.add_default_params(_params()))["stab_mu"] == 0.0
+    p = _params()
+    p["bcs"]["stabilization"] = {"domain": None}
+    with pytest.raises(ValueError, match="mandatory"):
+        g.add_default_params(p)
+    p = _params()
+    p["bcs"]["stabilization"] = {"μ": 0.3, "domain": "noslip"}
+    with pytest.raises(NotImplementedError):
+        g.add_default_params(p)
+
+
 def test_main_debug_mode_builds_spaces_only():
     xh, full, out = g.main(_params(debug=True))
     sp = xh.spaces
