@@ -155,6 +155,9 @@ struct CoarseLink {
   double *d_Ainv = nullptr;        // [n1c][n1c]
   double *d_LU = nullptr;          // [n1c][n1c] LU factors (set-up scratch)
   int32_t *d_piv = nullptr, *d_info = nullptr;
+  double *d_gjw = nullptr;         // block Gauss-Jordan workspace (column and row panels, pivot block, check)
+  int32_t *d_gjflag = nullptr;     // set when a diagonal block of the block Gauss-Jordan has a zero pivot
+  int dense_fallbacks = 0;         // coarsest inversions redone with the pivoted LU
   double omega = 0.175;            // damping of the Schwarz smoother inside the V-cycle
   int step_mode = 1;               // coarse correction step: 0 full, 1 minimal residual
   void *rr = nullptr;              // [3 (n1 + 1)] fine residual, A11 yc, yc (T)

@@ -288,12 +288,149 @@ __global__ void k_set_diag(int64_t n, double v, double *__restrict__ D) {
     D[i + i * n] = v;
 }
 
-// dense column-major copy of the [u,j] block (the A11 prefix of each row)
+// dense row-major copy of the [u,j] block (the A11 prefix of each row)
 __global__ void k_dense_a11(int64_t n1, const int64_t *__restrict__ rowptr, const int64_t *__restrict__ split,
                             const int32_t *__restrict__ colind, const double *__restrict__ val,
                             double *__restrict__ D) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n1; i += (int64_t)gridDim.x * blockDim.x)
-    for (int64_t p = rowptr[i]; p < split[i]; ++p) D[i + (int64_t)colind[p] * n1] = val[p];
+    for (int64_t p = rowptr[i]; p < split[i]; ++p) D[i * n1 + colind[p]] = val[p];
+}
+
+// ---------------------------------------------------------------------------
+// Dense inverse of the coarsest [u,j] block by block Gauss-Jordan: for each
+// block column K of width BGJ_B, P = A_KK^-1 (one workgroup, Gauss-Jordan with
+// partial pivoting inside the block), then the rank-BGJ_B update
+//   A <- A - A[:,K] P A[K,:],  A[K,:] <- P A[K,:],  A[:,K] <- -A[:,K] P,  A_KK <- P
+// on FP64 GEMMs (2 n^3 flops in all, like an LU plus triangular solves, but in
+// n / BGJ_B wide GEMM steps instead of one panel kernel per column).  Pivoting
+// is inside the diagonal blocks only; the result is checked by a residual and
+// redone with the row-pivoted LU if it fails.
+constexpr int BGJ_B = 96;
+
+// One 1024-thread workgroup inverts the kb x kb (kb <= BGJ_B = 96) block; the
+// threads form a 32 x 32 grid, thread (ti, tj) owning the entries
+// (ti + 32a, tj + 32b), a, b < 3, of the block held in LDS.
+__global__ __launch_bounds__(1024) void k_block_inv(int64_t ld, int kb, const double *__restrict__ Mkk,
+                                                    double *__restrict__ P, int32_t *__restrict__ flag) {
+  static_assert(BGJ_B == 96, "the 32 x 32 thread grid covers 96 x 96 entries");
+  __shared__ double A[BGJ_B][BGJ_B + 1];
+  __shared__ double colk[BGJ_B];
+  __shared__ int perm[BGJ_B];
+  __shared__ int piv_row;
+  __shared__ double piv_val;
+  const int tid = threadIdx.x, ti = tid & 31, tj = tid >> 5;
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int i = ti + 32 * a, j = tj + 32 * b;
+      if (i < kb && j < kb) A[i][j] = Mkk[i + (int64_t)j * ld];
+    }
+  __syncthreads();
+  for (int k = 0; k < kb; ++k) {
+    if (tid < 64) {
+      double best = -1.0;
+      int bi = k;
+      for (int i = k + tid; i < kb; i += 64) {
+        const double v = fabs(A[i][k]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_down(best, off, 64);
+        const int oi = __shfl_down(bi, off, 64);
+        if (ov > best || (ov == best && oi < bi)) {
+          best = ov;
+          bi = oi;
+        }
+      }
+      if (tid == 0) {
+        piv_row = bi;
+        piv_val = A[bi][k];
+        perm[k] = bi;
+        if (!(best > 0.0)) *flag = 1;
+      }
+    }
+    __syncthreads();
+    const int p = piv_row;
+    const double pv = piv_val;
+    if (p != k && tid < kb) {
+      const double t = A[k][tid];
+      A[k][tid] = A[p][tid];
+      A[p][tid] = t;
+    }
+    __syncthreads();
+    if (!(pv != 0.0)) continue;  // uniform: the flag is set, the block is singular
+    if (tid < kb) {
+      colk[tid] = tid == k ? 0.0 : A[tid][k];
+      A[k][tid] = (tid == k ? 1.0 : A[k][tid]) / pv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int i = ti + 32 * a, j = tj + 32 * b;
+        if (i < kb && j < kb && i != k) {
+          if (j == k) A[i][k] = -colk[i] * A[k][k];
+          else A[i][j] -= colk[i] * A[k][j];
+        }
+      }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges, last first
+  for (int k = kb - 1; k >= 0; --k) {
+    const int p = perm[k];
+    if (p != k && tid < kb) {
+      const double t = A[tid][k];
+      A[tid][k] = A[tid][p];
+      A[tid][p] = t;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int i = ti + 32 * a, j = tj + 32 * b;
+      if (i < kb && j < kb) P[i + (int64_t)j * kb] = A[i][j];
+    }
+}
+
+// b_i = deterministic values in [-1/2, 1/2) + 1 (the check's right-hand side)
+__global__ void k_check_rhs(int64_t n, double *__restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = 1.0 + (double)((uint64_t)(i * 2654435761ull) % 1000ull) / 1000.0 - 0.5;
+}
+
+// out[0] = max |z - b|, out[1] = max |b| (one workgroup)
+__global__ __launch_bounds__(1024) void k_check_residual(int64_t n, const double *__restrict__ z,
+                                                         const double *__restrict__ b, double *__restrict__ out) {
+  __shared__ double r0[16], r1[16];
+  double m0 = 0.0, m1 = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    m0 = fmax(m0, fabs(z[i] - b[i]));
+    m1 = fmax(m1, fabs(b[i]));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    m0 = fmax(m0, __shfl_down(m0, off, 64));
+    m1 = fmax(m1, __shfl_down(m1, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    r0[threadIdx.x >> 6] = m0;
+    r1[threadIdx.x >> 6] = m1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      m0 = fmax(m0, r0[w]);
+      m1 = fmax(m1, r1[w]);
+    }
+    out[0] = fmax(m0, r0[0]);
+    out[1] = fmax(m1, r1[0]);
+  }
 }
 
 // coarse state: u by injection from the fine state, everything else zero
@@ -1693,37 +1830,88 @@ int precond_setup(mhd_ctx *c, int prec, const double *d_state) {
     return rc;
   }
   if (L->coarse_iters == 0 && !cc->coarse) {
-    // exact coarsest solve: dense LU of its [u,j] block, then the explicit
-    // inverse as the LU solve A^T X = I (trsm-based, GEMM-rich): X = A^-T
-    // column-major is A^-1 row-major, the layout of the row-dot product
+    // exact coarsest solve: the explicit inverse of its [u,j] block, row-major
+    // (the layout of the row-dot product).  The row-major copy of A is the
+    // column-major A^T; block Gauss-Jordan inverts it in place into (A^T)^-1,
+    // whose column-major storage is A^-1 row-major.
     const int64_t m = L->n1c;
     rocblas_handle rb = cc->solver->rb;  // bound to cs, the stream of every level
+    const int64_t nw = 2 * m * BGJ_B + (int64_t)BGJ_B * BGJ_B + 3 * m + 2;
     if (!L->d_Ainv) {
       MHD_HIP(hipMalloc(&L->d_Ainv, sizeof(double) * (size_t)(m * m)));
       MHD_HIP(hipMalloc(&L->d_LU, sizeof(double) * (size_t)(m * m)));
       MHD_HIP(hipMalloc(&L->d_piv, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)));
       MHD_HIP(hipMalloc(&L->d_info, sizeof(int32_t)));
+      MHD_HIP(hipMalloc(&L->d_gjw, sizeof(double) * (size_t)nw));
+      MHD_HIP(hipMalloc(&L->d_gjflag, sizeof(int32_t)));
     }
     MHD_HIP(hipMemsetAsync(L->d_LU, 0, sizeof(double) * (size_t)(m * m), cs));
     hipLaunchKernelGGL(k_dense_a11, dim3(nblocks(m)), dim3(TB), 0, cs, m, cc->d_rowptr, cc->solver->d_split,
                        cc->d_colind, cc->d_val, L->d_LU);
-    MHD_HIP(hipMemsetAsync(L->d_Ainv, 0, sizeof(double) * (size_t)(m * m), cs));
-    hipLaunchKernelGGL(k_set_diag, dim3(nblocks(m)), dim3(TB), 0, cs, m, 1.0, L->d_Ainv);
     MHD_HIP(hipGetLastError());
-    if (rocsolver_dgetrf(rb, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m, L->d_piv, L->d_info) !=
-            rocblas_status_success ||
-        rocsolver_dgetrs(rb, rocblas_operation_transpose, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m,
-                         L->d_piv, L->d_Ainv, (rocblas_int)m) != rocblas_status_success) {
-      c->err = "rocsolver LU of the coarsest level failed";
+    MHD_HIP(hipMemcpyAsync(L->d_Ainv, L->d_LU, sizeof(double) * (size_t)(m * m), hipMemcpyDeviceToDevice, cs));
+    MHD_HIP(hipMemsetAsync(L->d_gjflag, 0, sizeof(int32_t), cs));
+    double *M = L->d_Ainv, *C = L->d_gjw, *R = C + m * BGJ_B, *P = R + m * BGJ_B;
+    double *vb = P + (int64_t)BGJ_B * BGJ_B, *vy = vb + m, *vz = vy + m, *chk = vz + m;
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    const rocblas_operation N = rocblas_operation_none;
+    bool blas_ok = true;
+    for (int64_t k0 = 0; k0 < m && blas_ok; k0 += BGJ_B) {
+      const int kb = (int)std::min<int64_t>(BGJ_B, m - k0);
+      const rocblas_int rm = (rocblas_int)m, rkb = kb;
+      hipLaunchKernelGGL(k_block_inv, dim3(1), dim3(1024), 0, cs, m, kb, M + k0 + k0 * m, P, L->d_gjflag);
+      MHD_HIP(hipGetLastError());
+      MHD_HIP(hipMemcpyAsync(C, M + k0 * m, sizeof(double) * (size_t)(m * kb), hipMemcpyDeviceToDevice, cs));
+      blas_ok = rocblas_dgemm(rb, N, N, rkb, rm, rkb, &one, P, rkb, M + k0, rm, &zero, R, rkb) ==
+                    rocblas_status_success &&
+                rocblas_dgemm(rb, N, N, rm, rm, rkb, &mone, C, rm, R, rkb, &one, M, rm) == rocblas_status_success;
+      if (!blas_ok) break;
+      MHD_HIP(hipMemcpy2DAsync(M + k0, sizeof(double) * m, R, sizeof(double) * kb, sizeof(double) * kb, m,
+                               hipMemcpyDeviceToDevice, cs));
+      blas_ok = rocblas_dgemm(rb, N, N, rm, rkb, rkb, &mone, C, rm, P, rkb, &zero, M + k0 * m, rm) ==
+                rocblas_status_success;
+      MHD_HIP(hipMemcpy2DAsync(M + k0 + k0 * m, sizeof(double) * m, P, sizeof(double) * kb, sizeof(double) * kb, kb,
+                               hipMemcpyDeviceToDevice, cs));
+    }
+    if (!blas_ok) {
+      c->err = "rocblas_dgemm of the coarsest block Gauss-Jordan failed";
       return -2;
     }
-    count(c, OP_SETUP_DENSE3, (double)m * m * m);
-    int32_t info = 0;
-    MHD_HIP(hipMemcpyAsync(&info, L->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, cs));
+    // check: z = A (A^-1 b) against b for a fixed b
+    hipLaunchKernelGGL(k_check_rhs, dim3(nblocks(m)), dim3(TB), 0, cs, m, vb);
+    hipLaunchKernelGGL((k_dense_rowdot<double>), dim3((unsigned)((m + 3) / 4)), dim3(256), 0, cs, m, M, vb, vy);
+    hipLaunchKernelGGL((k_dense_rowdot<double>), dim3((unsigned)((m + 3) / 4)), dim3(256), 0, cs, m, L->d_LU, vy,
+                       vz);
+    hipLaunchKernelGGL(k_check_residual, dim3(1), dim3(1024), 0, cs, m, vz, vb, chk);
+    MHD_HIP(hipGetLastError());
+    double hchk[2] = {0.0, 1.0};
+    int32_t flag = 0;
+    MHD_HIP(hipMemcpyAsync(hchk, chk, sizeof(hchk), hipMemcpyDeviceToHost, cs));
+    MHD_HIP(hipMemcpyAsync(&flag, L->d_gjflag, sizeof(int32_t), hipMemcpyDeviceToHost, cs));
     MHD_HIP(hipStreamSynchronize(cs));
-    if (info != 0) {
-      c->err = "coarsest [u,j] block is singular (getrf info " + std::to_string(info) + ")";
-      return -3;
+    count(c, OP_SETUP_DENSE3, (double)m * m * m);
+    if (flag || !(hchk[0] <= 1e-6 * hchk[1])) {
+      // redo with the row-pivoted LU: getrf of A^T (the row-major copy), then
+      // A^T X = I gives X = A^-T column-major = A^-1 row-major
+      ++L->dense_fallbacks;
+      count(c, OP_SETUP_DENSE3, (double)m * m * m);  // a second inversion
+      MHD_HIP(hipMemsetAsync(M, 0, sizeof(double) * (size_t)(m * m), cs));
+      hipLaunchKernelGGL(k_set_diag, dim3(nblocks(m)), dim3(TB), 0, cs, m, 1.0, M);
+      MHD_HIP(hipGetLastError());
+      if (rocsolver_dgetrf(rb, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m, L->d_piv, L->d_info) !=
+              rocblas_status_success ||
+          rocsolver_dgetrs(rb, rocblas_operation_none, (rocblas_int)m, (rocblas_int)m, L->d_LU, (rocblas_int)m,
+                           L->d_piv, M, (rocblas_int)m) != rocblas_status_success) {
+        c->err = "rocsolver LU of the coarsest level failed";
+        return -2;
+      }
+      int32_t info = 0;
+      MHD_HIP(hipMemcpyAsync(&info, L->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, cs));
+      MHD_HIP(hipStreamSynchronize(cs));
+      if (info != 0) {
+        c->err = "coarsest [u,j] block is singular (getrf info " + std::to_string(info) + ")";
+        return -3;
+      }
     }
   }
   return 0;
@@ -1827,7 +2015,8 @@ void mhd_coarse_free(mhd_ctx *c) {
   if (!c || !c->coarse) return;
   CoarseLink *L = c->coarse;
   void *ptrs[] = {L->d_Prow, L->d_Trow, L->d_Pcol, L->d_Tcol, L->d_Pval, L->d_Tval, L->d_Pval32, L->d_Tval32,
-                  L->d_Rrow, L->d_Rcol, L->d_Rval, L->rr, L->rc, L->ec, L->d_Ainv, L->d_LU, L->d_piv, L->d_info, L->d_ab};
+                  L->d_Rrow, L->d_Rcol, L->d_Rval, L->rr, L->rc, L->ec, L->d_Ainv, L->d_LU, L->d_piv, L->d_info, L->d_ab,
+                  L->d_gjw, L->d_gjflag};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (L->ctx) mhd_destroy(L->ctx);

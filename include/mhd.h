@@ -283,7 +283,8 @@ int mhd_smoother_info(mhd_ctx *ctx, int64_t *npatch, int64_t *patch_dim, int64_t
  *      linear combinations, minimal-residual coarse steps)
  *   7  cells of the coarse-level Jacobian assemblies (preconditioner set-up)
  *   8  sum of dim^3 over the inverted patches (set-up)
- *   9  sum of n^3 over the dense coarsest inversions (set-up)
+ *   9  sum of n^3 over the dense coarsest inversions (set-up; an inverse whose
+ *      block Gauss-Jordan check fails and is redone by the pivoted LU counts twice)
  *   10 outer FGMRES iterations
  * reset != 0 zeroes the counts after reading them. */
 #define MHD_NOPS 11

@@ -87,6 +87,12 @@ def test_multilevel_solve_matches_direct(nc, levels, coarse_iters, step, precisi
         _say(f"  newton={info.newton_iters} krylov={info.krylov_iters} rc={rc} err={err['all']:.2e}")
         assert rc == 0 and info.converged
         assert err["all"] < (1e-10 if precision == 0 else 1e-8), err
+        if coarse_iters == 0:
+            # one dense coarsest inverse per Newton step, by block Gauss-Jordan
+            # (its residual check passed: a pivoted-LU redo would count twice)
+            coarsest = sp._multigrid_cache[(0, 1)]["chain"][levels - 1]
+            n1c = coarsest.nfree["u"] + coarsest.nfree["j"]
+            assert ctx.op_counts()["setup_dense_dim3"] == info.newton_iters * float(n1c) ** 3
     finally:
         ctx.close()
 
