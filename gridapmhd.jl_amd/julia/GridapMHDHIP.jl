@@ -42,6 +42,9 @@ struct MhdMesh
   ncells::Int32; nfaces_tw::Int32; cell_xyz::Ptr{Float64}; cell_verts::Ptr{Int32}; nverts::Int32; pad::Int32
   tw_face_cell::Ptr{Int32}; tw_face_lface::Ptr{Int8}; tw_tau::Ptr{Float64}; tw_cw::Ptr{Float64}; tw_jw::Ptr{Float64}
 end
+struct MhdSkeleton
+  nfaces::Int32; pad::Int32; face_cell::Ptr{Int32}; face_lface::Ptr{Int8}; face_perm::Ptr{Int8}; mu::Ptr{Float64}
+end
 struct MhdSpaces
   nfree::NTuple{4,Int32}; ndir::Int32; block_style::Int32
   cell_dofs::Ptr{Int32}; rt_sign::Ptr{Int8}; dir_vals::Ptr{Float64}
@@ -73,7 +76,7 @@ function GridapMHD.default_solver_params(::Val{:hip})
     :vector_type => Vector{Float64},
     :solver_postpro => ((cache, info) -> hip_postpro(cache, info)),
     :niter => 20, :niter_ls => 400, :rtol => 1e-6, :atol => 1e-8,
-    :precision => :fp64,           # :fp64 | :mixed | :mixed_bf16
+    :precision => :fp64,           # :fp64 | :mixed | :mixed_bf16 | :mixed_fp16
     :inner_iters => 3, :restart => 200,
     :levels => :auto,              # [u,j] V-cycle levels (x/y semi-coarsening), see attach_levels!
     :coarse_iters => 0,            # 0: exact coarsest solve
@@ -98,12 +101,18 @@ function GridapMHD._solver(::Val{:hip}, op, params)
   U = get_trial(op)
   data = extract_problem(U, params)             # arrays below, built once
   ctx = Ref{Ptr{Cvoid}}(C_NULL)
-  rc = ccall((:mhd_create, libmhd), Cint,
-             (Ptr{Ptr{Cvoid}}, Ref{MhdMesh}, Ref{MhdSpaces}, Ref{MhdParams}, Int32),
-             ctx, data.mesh, data.spaces, data.params, Int32(0))
+  if data.skeleton.nfaces > 0               # params[:bcs][:stabilization]
+    rc = ccall((:mhd_create_stabilized, libmhd), Cint,
+               (Ptr{Ptr{Cvoid}}, Ref{MhdMesh}, Ref{MhdSpaces}, Ref{MhdParams}, Ref{MhdSkeleton}, Int32),
+               ctx, data.mesh, data.spaces, data.params, data.skeleton, Int32(0))
+  else
+    rc = ccall((:mhd_create, libmhd), Cint,
+               (Ptr{Ptr{Cvoid}}, Ref{MhdMesh}, Ref{MhdSpaces}, Ref{MhdParams}, Int32),
+               ctx, data.mesh, data.spaces, data.params, Int32(0))
+  end
   check(rc, ctx[], "mhd_create")
   s = params[:solver]
-  prec = Dict(:fp64 => 0, :mixed => 1, :mixed_bf16 => 2)[s[:precision]]
+  prec = Dict(:fp64 => 0, :mixed => 1, :mixed_bf16 => 2, :mixed_fp16 => 3)[s[:precision]]
   opts = MhdSolverOpts(s[:niter], s[:restart], s[:niter_ls], prec, s[:rtol], s[:atol], s[:rtol] / 10,
                        s[:inner_iters], 0)
   solver = HIPNewtonSolver(ctx, opts, data.to_mhd, data.from_mhd, data.covec_to_mhd, data.keep)
@@ -261,10 +270,6 @@ are those of Gridap 0.17 / 0.18; the face-orientation signs of the RT space
 are read from `get_sign_flip`, which assumes Gridap's own orientation rule.
 """
 function extract_problem(U, params)
-  # the skeleton stabilisation needs the interior-face list of mhd_create_stabilized,
-  # which this glue does not extract yet
-  isempty(get(params[:bcs], :stabilization, [])) ||
-    error("GridapMHDHIP: params[:bcs][:stabilization] is not passed to libmhd by this glue yet")
   model = params[:model]
   Ω = Triangulation(model)
   ncells = num_cells(Ω)
@@ -387,6 +392,42 @@ function extract_problem(U, params)
       end
     end
   end
+  # skeleton stabilisation (src/weakforms.jl:168-177, 685-688): every face of two
+  # cells, the second cell's 3x3 face Gauss point at each point of the first,
+  # matched through the face's corner vertex ids (periodic images identified)
+  sk_cell, sk_lf, sk_perm, sk_mu = Int32[], Int8[], Int8[], Float64[]
+  stab = get(params[:bcs], :stabilization, [])
+  stab isa AbstractDict && (stab = [stab])
+  if !isempty(stab)
+    μ = sum(Float64(st[:μ]) for st in stab)
+    c2f = get_faces(topo, 3, 2)
+    f2c = get_faces(topo, 2, 3)
+    # corner (i0, i1) of local face lf: local vertex side << d | i0 << ta | i1 << tb
+    function corner_vertex(lf, k)
+      d = 2 - lf ÷ 2
+      ta, tb = d == 0 ? (1, 2) : (d == 1 ? (0, 2) : (0, 1))
+      ((lf & 1) << d) | ((k & 1) << ta) | ((k >> 1) << tb)
+    end
+    localface(c, f) = findfirst(k -> c2f.data[c2f.ptrs[c] + k - 1] == f, 1:6) - 1
+    bit(v, i) = (v >> i) & 1
+    for f in 1:num_faces(topo, 2)
+      cs = f2c.data[f2c.ptrs[f]:f2c.ptrs[f+1]-1]
+      length(cs) == 2 || continue
+      c0, c1 = cs
+      l0, l1 = localface(c0, f), localface(c1, f)
+      P = [cell_verts[8(c0 - 1) + corner_vertex(l0, k) + 1] for k in 0:3]
+      M = [cell_verts[8(c1 - 1) + corner_vertex(l1, k) + 1] for k in 0:3]
+      m = [findfirst(==(P[k]), M) - 1 for k in 1:4]     # minus corner of each plus corner
+      push!(sk_cell, c0 - 1, c1 - 1); push!(sk_lf, l0, l1); push!(sk_mu, μ)
+      for q in 0:8
+        qa, qb = q % 3, q ÷ 3
+        s0 = 2bit(m[1], 0) + (bit(m[2], 0) - bit(m[1], 0)) * qa + (bit(m[3], 0) - bit(m[1], 0)) * qb
+        s1 = 2bit(m[1], 1) + (bit(m[2], 1) - bit(m[1], 1)) * qa + (bit(m[3], 1) - bit(m[1], 1)) * qb
+        push!(sk_perm, s0 + 3s1)
+      end
+    end
+  end
+  skeleton = MhdSkeleton(length(sk_mu), 0, pointer(sk_cell), pointer(sk_lf), pointer(sk_perm), pointer(sk_mu))
   f = params[:fluid]
   conv = Dict(:none => 0, :picard => 1, :newton => 2)[f[:convection]]
   B, fv = Tuple(Float64.(f[:B])), Tuple(Float64.(f[:f]))
@@ -395,12 +436,13 @@ function extract_problem(U, params)
   # Gridap's dof values carry the RT orientation; libmhd's local face signs are the same flips
   rt_sign = Int8[flip[c][4(k - 1) + 1] > 0 ? 1 : -1 for k in 1:6 for c in 1:ncells]
   rt_sign = reshape(permutedims(reshape(rt_sign, ncells, 6)), :)
-  keep = Any[cell_xyz, cell_verts, cell_dofs, dir_vals, rt_sign, tw_cell, tw_lf, tw_tau, tw_cw, tw_jw]
+  keep = Any[cell_xyz, cell_verts, cell_dofs, dir_vals, rt_sign, tw_cell, tw_lf, tw_tau, tw_cw, tw_jw,
+             sk_cell, sk_lf, sk_perm, sk_mu]
   mesh = MhdMesh(ncells, length(tw_cell), pointer(cell_xyz), pointer(cell_verts), nverts, 0,
                  pointer(tw_cell), pointer(tw_lf), pointer(tw_tau), pointer(tw_cw), pointer(tw_jw))
   spaces = MhdSpaces(Int32.((nfree[1], nfree[2], nfree[3], nfree[4])), Int32(length(dir_vals)), Int32(1),
                      pointer(cell_dofs), pointer(rt_sign), pointer(dir_vals))
-  (mesh = mesh, spaces = spaces, params = mparams, to_mhd = to_mhd, from_mhd = from_mhd,
+  (mesh = mesh, spaces = spaces, params = mparams, skeleton = skeleton, to_mhd = to_mhd, from_mhd = from_mhd,
    covec_to_mhd = covec_to_mhd, keep = keep)
 end
 
