@@ -311,7 +311,12 @@ int mhd_create_stabilized(mhd_ctx **out, const mhd_mesh *mesh, const mhd_spaces 
     std::vector<int32_t> sentry(std::max<int32_t>(srow[n], 1)), fill(srow.begin(), srow.end() - 1);
     for (int64_t k = 0; k < (int64_t)sdofs.size(); ++k)
       if (sdofs[k] >= 0) sentry[fill[sdofs[k]]++] = (int32_t)k;
+    if (c->max_row >= mhd::RANK_NONE) {
+      c->err = "skeleton: a CSR row of " + std::to_string(c->max_row) + " entries exceeds the 16-bit rank table";
+      return -1;
+    }
     c->nskel = nskel;
+    c->nsentry = srow[n];
     if ((rc = dev_upload(c, &c->d_skel_dofs, sdofs.data(), sdofs.size()))) return rc;
     if ((rc = dev_upload(c, &c->d_srow, srow.data(), srow.size()))) return rc;
     if ((rc = dev_upload(c, &c->d_sentry, sentry.data(), sentry.size()))) return rc;
@@ -352,7 +357,7 @@ void mhd_destroy(mhd_ctx *c) {
   void *ptrs[] = {c->d_xyz, c->d_dofs, c->d_sign, c->d_dir, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw,
                   c->d_rowptr, c->d_colind, c->d_rank, c->d_val, c->d_tperm, c->d_ref, c->d_fref, c->d_x, c->d_r,
                   c->d_color_cells, c->d_crow, c->d_centry, c->d_ebuf, c->d_rbuf, c->d_skel_dofs, c->d_skel_K,
-                  c->d_srow, c->d_sentry};
+                  c->d_srow, c->d_sentry, c->d_skel_rank, c->d_skel_res};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);

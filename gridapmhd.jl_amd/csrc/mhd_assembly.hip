@@ -1112,43 +1112,96 @@ __global__ __launch_bounds__(64) void k_skel_matrix(int nf, const double *__rest
   }
 }
 
-// Skeleton rows of the residual: r_i += sum over the row's (face, component,
-// slot) entries of K_f[slot][:] . x at the face's slot dofs of that component
-// (Dirichlet values for non-free slots).  One thread per row, entries in
+// Skeleton residual, in two steps.  k_skel_face_product: one wave per face,
+// lane a < 54 forms (K_f x_c)[a] for the three components c from the slot
+// values x_c (Dirichlet values for non-free slots, 0 for unused ones), reading
+// K_f by rows b (K_f is symmetric bit for bit: its entries are sums of the
+// same products in the same order), so the loads are coalesced.
+// k_skel_res_gather: one thread per row adds the face products of the row's
+// (face, component, slot) entries, whose index is the product's own index, in
 // increasing face order: the sums do not depend on the launch.
-__global__ void k_skel_residual(int64_t n, const int32_t *__restrict__ srow, const int32_t *__restrict__ sentry,
-                                const int32_t *__restrict__ sdofs, const double *__restrict__ K,
-                                const double *__restrict__ x, const double *__restrict__ dir, double *__restrict__ r) {
+__global__ __launch_bounds__(64) void k_skel_face_product(int nf, const int32_t *__restrict__ sdofs,
+                                                          const double *__restrict__ K, const double *__restrict__ x,
+                                                          const double *__restrict__ dir, double *__restrict__ fres) {
+  __shared__ double xs[SKEL_DOFS];
+  const int lane = threadIdx.x;
+  for (int f = blockIdx.x; f < nf; f += gridDim.x) {
+    for (int e = lane; e < SKEL_DOFS; e += 64) {
+      const int32_t g = sdofs[(int64_t)f * SKEL_DOFS + e];
+      xs[e] = g == SKEL_NONE ? 0.0 : (g >= 0 ? x[g] : dir[-(int64_t)g - 1]);
+    }
+    __syncthreads();
+    if (lane < SKEL_SLOTS) {
+      const double *Kf = K + (int64_t)f * SKEL_SLOTS * SKEL_SLOTS;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+      for (int b = 0; b < SKEL_SLOTS; ++b) {
+        const double k = Kf[b * SKEL_SLOTS + lane];
+        a0 += k * xs[b];
+        a1 += k * xs[SKEL_SLOTS + b];
+        a2 += k * xs[2 * SKEL_SLOTS + b];
+      }
+      double *o = fres + (int64_t)f * SKEL_DOFS;
+      o[lane] = a0;
+      o[SKEL_SLOTS + lane] = a1;
+      o[2 * SKEL_SLOTS + lane] = a2;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_skel_res_gather(int64_t n, const int32_t *__restrict__ srow, const int32_t *__restrict__ sentry,
+                                  const double *__restrict__ fres, double *__restrict__ r) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int p0 = srow[i], p1 = srow[i + 1];
     if (p0 == p1) continue;
     double acc = 0.0;
-    for (int p = p0; p < p1; ++p) {
-      const int32_t e = sentry[p];
-      const int f = e / SKEL_DOFS, cs = e % SKEL_DOFS;
-      const int comp = cs / SKEL_SLOTS, a = cs % SKEL_SLOTS;
-      const int32_t *sd = sdofs + (int64_t)f * SKEL_DOFS + comp * SKEL_SLOTS;
-      const double *Kr = K + ((int64_t)f * SKEL_SLOTS + a) * SKEL_SLOTS;
-      for (int b = 0; b < SKEL_SLOTS; ++b) {
-        const int32_t g = sd[b];
-        if (g == SKEL_NONE) continue;
-        acc += Kr[b] * (g >= 0 ? x[g] : dir[-(int64_t)g - 1]);
-      }
-    }
+    for (int p = p0; p < p1; ++p) acc += fres[sentry[p]];
     r[i] += acc;
+  }
+}
+
+// Rank of every slot dof of an entry's face (same component) in the entry's
+// CSR row, found once at creation by binary search (RANK_NONE for non-free
+// slots).  One wave per row.
+__global__ __launch_bounds__(64) void k_skel_rank(int64_t n, const int64_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ colind, const int32_t *__restrict__ srow,
+                                                  const int32_t *__restrict__ sentry,
+                                                  const int32_t *__restrict__ sdofs, uint16_t *__restrict__ rank) {
+  const int lane = threadIdx.x;
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const int p0 = srow[i], p1 = srow[i + 1];
+    if (p0 == p1) continue;
+    const int64_t rb = rowptr[i];
+    const int len = (int)(rowptr[i + 1] - rb);
+    for (int p = p0; p < p1; ++p) {
+      if (lane >= SKEL_SLOTS) continue;
+      const int32_t e = sentry[p];
+      const int f = e / SKEL_DOFS, comp = (e % SKEL_DOFS) / SKEL_SLOTS;
+      const int32_t g = sdofs[(int64_t)f * SKEL_DOFS + comp * SKEL_SLOTS + lane];
+      uint16_t rk = RANK_NONE;
+      if (g >= 0) {
+        int lo = 0, hi = len;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (colind[rb + mid] < g) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo < len && colind[rb + lo] == g) rk = (uint16_t)lo;
+      }
+      rank[(int64_t)p * SKEL_SLOTS + lane] = rk;
+    }
   }
 }
 
 // Skeleton rows of the Jacobian, added to the assembled CSR values: one
 // 64-lane wave per row; the row is accumulated in LDS (lane b adds K_f[slot][b]
-// at the rank of slot b's dof, found by binary search in the sorted row --
-// the slot dofs of one entry are distinct, so lanes never collide), entries
-// in increasing face order, then added to the row once.
+// at the rank of slot b's dof, from the rank table -- the slot dofs of one
+// entry are distinct, so lanes never collide), entries in increasing face
+// order, then added to the row once.
 __global__ __launch_bounds__(64) void k_skel_jacobian(int64_t n, const int64_t *__restrict__ rowptr,
-                                                      const int32_t *__restrict__ colind,
                                                       const int32_t *__restrict__ srow,
                                                       const int32_t *__restrict__ sentry,
-                                                      const int32_t *__restrict__ sdofs,
+                                                      const uint16_t *__restrict__ rank,
                                                       const double *__restrict__ K, double *__restrict__ val) {
   extern __shared__ double sk_lds[];
   const int lane = threadIdx.x;
@@ -1160,20 +1213,12 @@ __global__ __launch_bounds__(64) void k_skel_jacobian(int64_t n, const int64_t *
     for (int t = lane; t < len; t += 64) sk_lds[t] = 0.0;
     __builtin_amdgcn_wave_barrier();
     for (int p = p0; p < p1; ++p) {
-      const int32_t e = sentry[p];
-      const int f = e / SKEL_DOFS, cs = e % SKEL_DOFS;
-      const int comp = cs / SKEL_SLOTS, a = cs % SKEL_SLOTS;
       if (lane < SKEL_SLOTS) {
-        const int32_t g = sdofs[(int64_t)f * SKEL_DOFS + comp * SKEL_SLOTS + lane];
-        if (g >= 0) {
-          int lo = 0, hi = len;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (colind[rb + mid] < g) lo = mid + 1;
-            else hi = mid;
-          }
-          if (lo < len && colind[rb + lo] == g)
-            sk_lds[lo] += K[((int64_t)f * SKEL_SLOTS + a) * SKEL_SLOTS + lane];
+        const uint16_t rk = rank[(int64_t)p * SKEL_SLOTS + lane];
+        if (rk != RANK_NONE) {
+          const int32_t e = sentry[p];
+          const int f = e / SKEL_DOFS, a = (e % SKEL_DOFS) % SKEL_SLOTS;
+          sk_lds[rk] += K[((int64_t)f * SKEL_SLOTS + a) * SKEL_SLOTS + lane];
         }
       }
       __builtin_amdgcn_wave_barrier();
@@ -1235,18 +1280,27 @@ int mhd_build_skeleton(mhd_ctx *c, const int32_t *d_fcell, const int8_t *d_flf, 
                        const uint8_t *d_mslot, const double *d_mu) {
   if (c->nskel <= 0) return 0;
   MHD_HIP(hipMalloc(&c->d_skel_K, sizeof(double) * (size_t)c->nskel * SKEL_SLOTS * SKEL_SLOTS));
+  MHD_HIP(hipMalloc(&c->d_skel_res, sizeof(double) * (size_t)c->nskel * SKEL_DOFS));
+  MHD_HIP(hipMalloc(&c->d_skel_rank, sizeof(uint16_t) * (size_t)std::max<int64_t>(c->nsentry, 1) * SKEL_SLOTS));
   const int blocks = std::min(c->nskel, 1 << 20);
   hipLaunchKernelGGL(k_skel_matrix, dim3(blocks), dim3(64), 0, c->stream, c->nskel, c->d_xyz, d_fcell, d_flf,
                      d_fperm, d_mslot, d_mu, c->d_fref, c->d_skel_K);
   MHD_HIP(hipGetLastError());
+  if (c->n > 0) {
+    hipLaunchKernelGGL(k_skel_rank, dim3((unsigned)std::min<int64_t>(c->n, 1 << 20)), dim3(64), 0, c->stream, c->n,
+                       c->d_rowptr, c->d_colind, c->d_srow, c->d_sentry, c->d_skel_dofs, c->d_skel_rank);
+    MHD_HIP(hipGetLastError());
+  }
   return 0;
 }
 
 int mhd_launch_skeleton_residual(mhd_ctx *c, const double *d_x, double *d_r, hipStream_t s) {
   if (c->nskel <= 0 || c->n <= 0) return 0;
+  hipLaunchKernelGGL(k_skel_face_product, dim3((unsigned)std::min(c->nskel, 1 << 20)), dim3(64), 0, s, c->nskel,
+                     c->d_skel_dofs, c->d_skel_K, d_x, c->d_dir, c->d_skel_res);
   const int64_t blocks = std::min<int64_t>((c->n + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_skel_residual, dim3((unsigned)blocks), dim3(256), 0, s, c->n, c->d_srow, c->d_sentry,
-                     c->d_skel_dofs, c->d_skel_K, d_x, c->d_dir, d_r);
+  hipLaunchKernelGGL(k_skel_res_gather, dim3((unsigned)blocks), dim3(256), 0, s, c->n, c->d_srow, c->d_sentry,
+                     c->d_skel_res, d_r);
   MHD_HIP(hipGetLastError());
   return 0;
 }
@@ -1259,8 +1313,8 @@ int mhd_launch_skeleton_jacobian(mhd_ctx *c, double *d_val, hipStream_t s) {
     return -1;
   }
   const int64_t blocks = std::min<int64_t>(c->n, 1 << 20);
-  hipLaunchKernelGGL(k_skel_jacobian, dim3((unsigned)blocks), dim3(64), lds, s, c->n, c->d_rowptr, c->d_colind,
-                     c->d_srow, c->d_sentry, c->d_skel_dofs, c->d_skel_K, d_val);
+  hipLaunchKernelGGL(k_skel_jacobian, dim3((unsigned)blocks), dim3(64), lds, s, c->n, c->d_rowptr, c->d_srow,
+                     c->d_sentry, c->d_skel_rank, c->d_skel_K, d_val);
   MHD_HIP(hipGetLastError());
   return 0;
 }
@@ -1353,7 +1407,7 @@ int mhd_launch_residual_noalloc(mhd_ctx *c, const double *d_x, double *d_r, hipS
                        c->d_dir, d_x, c->d_fref, c->d_fcell, c->d_flf, c->d_ftau, c->d_fcw, c->d_fjw, d_r, 0);
     MHD_HIP(hipGetLastError());
   }
-  // the skeleton face matrices are context memory made at creation
+  // the skeleton face matrices and face products are context memory made at creation
   return mhd_launch_skeleton_residual(c, d_x, d_r, s);
 }
 

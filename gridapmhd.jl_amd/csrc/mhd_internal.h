@@ -131,6 +131,9 @@ struct mhd_ctx {
   double *d_skel_K = nullptr;         // [nskel][54][54] face matrices of one velocity component
   int32_t *d_srow = nullptr;          // [n + 1] skeleton entries of each row
   int32_t *d_sentry = nullptr;        // face * SKEL_DOFS + component * 54 + slot, increasing face order per row
+  int64_t nsentry = 0;                // entries over all rows
+  uint16_t *d_skel_rank = nullptr;    // [nsentry][54] rank of each slot dof of the entry's face in the entry's row
+  double *d_skel_res = nullptr;       // [nskel][SKEL_DOFS] the face products K_f x_c of the last residual
   // solver workspace (allocated lazily)
   struct Solver *solver = nullptr;
   // coarser level of the [u,j] block solve (owned), see mhd_attach_coarse
