@@ -187,7 +187,8 @@ int mhd_csr_arrays(mhd_ctx *ctx, const int64_t **d_rowptr, const int32_t **d_col
  * hipMalloc, hipFree or hipMemset.  At most 130,055 cells (16,777,216 / 129
  * element-residual doubles); larger meshes are rejected with an error.  Every
  * call on one GPU shares that GPU's workspace, so calls that overlap on one
- * GPU (any context, any stream) race. */
+ * GPU (any context, any stream) race.  A context made by mhd_create_stabilized
+ * adds its skeleton term through face-product memory allocated at creation. */
 int mhd_residual_d_noalloc(mhd_ctx *ctx, const double *d_x, double *d_r, void *stream);
 
 /* y = A x with the Jacobian last assembled into the context (device pointers). */
