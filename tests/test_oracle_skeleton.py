@@ -139,3 +139,51 @@ def test_unstructured_rotated_cells(orc_mod):
     want = 2.0 * MU * np.sum(np.outer(np.diff(ys), np.diff(zs)) ** 2)
     got = kx @ (K @ kx)
     assert abs(got - want) < 1e-11 * want, (got, want)
+
+
+@pytest.mark.parametrize("kind", ["periodic_tanh", "scrambled"])
+def test_product_face_point_permutation(orc_mod, kind):
+    """The product's face list (fespaces.skeleton_faces, found through corner
+    vertex ids) maps each 3x3 Gauss point of a face's first cell to the
+    coinciding point of the second: the physical positions agree (up to the
+    periodic translation), with the point order of the library's face tables
+    (lower tangential axis fastest)."""
+    import gridapmhd_amd as g
+    from test_mesh_io import _scramble
+    if kind == "periodic_tanh":
+        sp = cases.hunt_case(4, Ha=50.0)[0]
+    else:
+        cm = g.meshers.cartesian_model((0.0, 1.0, 0.0, 1.0, 0.0, 1.0), (3, 3, 2), (False, False, False), map=_stretch)
+        sp = g.fespaces.setup_fe_spaces(_scramble(g.mesh_io.hexmesh_from_cartesian(cm), 4), [], [],
+                                        style="badia2024")
+    sk = g.fespaces.skeleton_faces(sp)
+    X = sp.mesh.cell_xyz()
+    gpt, _ = orc_mod._gauss3()
+
+    def points(cells, lfs):
+        out = np.zeros((cells.size, 9, 3))
+        for lf in range(6):
+            m = lfs == lf
+            if not m.any():
+                continue
+            d = 2 - lf // 2
+            ta, tb = [a for a in range(3) if a != d]
+            xi = np.zeros((9, 3))
+            xi[:, d] = lf & 1
+            xi[:, ta] = np.tile(gpt, 3)
+            xi[:, tb] = np.repeat(gpt, 3)
+            for k in np.nonzero(m)[0]:
+                out[k] = orc_mod.trilinear_map(np.repeat(X[cells[k]][None], 9, 0), xi)[0]
+        return out
+
+    P0 = points(sk["cell"][:, 0], sk["lface"][:, 0])
+    P1 = points(sk["cell"][:, 1], sk["lface"][:, 1])
+    P1p = np.take_along_axis(P1, sk["perm"].astype(np.int64)[:, :, None], 1)
+    shift = P0.mean(1, keepdims=True) - P1p.mean(1, keepdims=True)
+    assert np.abs(P0 - (P1p + shift)).max() < 1e-12
+    # the shift is zero except across the periodic faces, where it is the period
+    per = np.abs(shift).max(axis=(1, 2)) > 1e-12
+    if kind == "periodic_tanh":
+        assert per.any() and np.allclose(np.abs(shift[per]).sum(axis=(1, 2)), 0.1)
+    else:
+        assert not per.any()
