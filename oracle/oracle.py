@@ -416,6 +416,24 @@ def null_pins(spaces, pin_p=True, pin_phi=False):
     return out
 
 
+def null_fields(orc: Oracle, spaces, x=None, tol=1e-12):
+    """The fields among p and phi whose constant is a null vector of the
+    Jacobian at x (A c = 0 for c = 1 on that field's dofs: the dP1 and dQ1
+    bases both sum to one), i.e. the fields ``null_pins`` should pin.  The
+    constant pressure is null with u Dirichlet on every wall; the constant
+    potential only where j.n is prescribed on the whole boundary."""
+    x = np.zeros(orc.nfree) if x is None else x
+    A = orc.jacobian(x)
+    scale = abs(A).max()
+    out = []
+    for f in ("p", "phi"):
+        c = np.zeros(orc.nfree)
+        c[spaces.field_slices()[f]] = 1.0
+        if np.abs(A @ c).max() <= tol * scale * max(1, np.sqrt(c.sum())):
+            out.append(f)
+    return out
+
+
 def direct_solve(A: sps.csc_matrix, b: np.ndarray, pins=()) -> np.ndarray:
     """Solve A x = b with SuperLU; rows/cols of ``pins`` replaced by identity."""
     if pins:
