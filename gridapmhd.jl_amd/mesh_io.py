@@ -135,6 +135,8 @@ def read_gmsh(path: str) -> HexMesh:
         if param:
             raise ValueError(f"{path}: parametric node coordinates are not supported")
         k += 1
+        if m == 0:                      # an entity without nodes of its own
+            continue
         tag_of[n:n + m] = [int(lines[k + q]) for q in range(m)]
         k += m
         xyz[n:n + m] = [[float(v) for v in lines[k + q].split()[:3]] for q in range(m)]
@@ -154,6 +156,9 @@ def read_gmsh(path: str) -> HexMesh:
     for _ in range(nblocks):
         d, etag, etype, m = (int(v) for v in lines[k].split())
         k += 1
+        if etype in (2, 4, 6, 7):
+            raise ValueError(f"{path}: triangle / tetrahedron / prism / pyramid elements (gmsh type {etype}): the "
+                             "hexahedral Q2 / dP1 / RT1 / dQ1 discretisation needs an all-hexahedral mesh")
         if etype not in _GMSH_NNODES:
             raise ValueError(f"{path}: element type {etype} is not supported (8-node hexahedra meshes only)")
         for q in range(m):

@@ -201,3 +201,29 @@ def test_expansion_dirichlet_values_follow_tag_order():
     assert np.array_equal(got[on_in], want_in[on_in]) and np.abs(want_in[on_in]).max() > 5.0
     assert not got[~on_in].any()
     assert sp.ndir["j"] == 4 * len(sp.mesh.quad_entity) and not sp.dir_vals[sp.ndir["u"]:].any()
+
+
+def test_gmsh_reader_edge_cases(tmp_path):
+    """An entity block without nodes (gmsh writes them for entities whose nodes
+    all belong to lower-dimensional ones) is skipped; simplex elements are
+    rejected with a message naming the hexahedral discretisation."""
+    from gridapmhd_amd import mesh_io
+    hm = mesh_io.hexmesh_from_cartesian(_box())
+    path = str(tmp_path / "box.msh")
+    mesh_io.write_gmsh(path, hm)
+    lines = open(path).read().split("\n")
+    i = lines.index("$Nodes")
+    h = lines[i + 1].split()
+    lines[i + 1] = " ".join([str(int(h[0]) + 1)] + h[1:])
+    lines.insert(i + 2, "3 1 0 0")                       # volume entity 1, no nodes of its own
+    open(path, "w").write("\n".join(lines))
+    back = mesh_io.read_gmsh(path)
+    assert np.array_equal(back.cells, hm.cells) and np.array_equal(back.node_xyz, hm.node_xyz)
+    j = lines.index("$Elements")
+    h = lines[j + 1].split()
+    lines[j + 1] = " ".join([str(int(h[0]) + 1), str(int(h[1]) + 1)] + h[2:])
+    lines.insert(j + 2, "3 1 4 1")
+    lines.insert(j + 3, "999999 1 2 3 4")
+    open(path, "w").write("\n".join(lines))
+    with pytest.raises(ValueError, match="all-hexahedral"):
+        mesh_io.read_gmsh(path)
