@@ -405,18 +405,21 @@ __global__ void k_check_rhs(int64_t n, double *__restrict__ b) {
     b[i] = 1.0 + (double)((uint64_t)(i * 2654435761ull) % 1000ull) / 1000.0 - 0.5;
 }
 
-// out[0] = max |z - b|, out[1] = max |b| (one workgroup)
+// max that keeps a NaN (fmax would drop it): a NaN in the check fails it
+__device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// out[0] = max |z - b|, out[1] = max |b| (one workgroup); NaN if z has one
 __global__ __launch_bounds__(1024) void k_check_residual(int64_t n, const double *__restrict__ z,
                                                          const double *__restrict__ b, double *__restrict__ out) {
   __shared__ double r0[16], r1[16];
   double m0 = 0.0, m1 = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-    m0 = fmax(m0, fabs(z[i] - b[i]));
-    m1 = fmax(m1, fabs(b[i]));
+    m0 = nan_max(m0, fabs(z[i] - b[i]));
+    m1 = nan_max(m1, fabs(b[i]));
   }
   for (int off = 32; off > 0; off >>= 1) {
-    m0 = fmax(m0, __shfl_down(m0, off, 64));
-    m1 = fmax(m1, __shfl_down(m1, off, 64));
+    m0 = nan_max(m0, __shfl_down(m0, off, 64));
+    m1 = nan_max(m1, __shfl_down(m1, off, 64));
   }
   if ((threadIdx.x & 63) == 0) {
     r0[threadIdx.x >> 6] = m0;
@@ -425,11 +428,11 @@ __global__ __launch_bounds__(1024) void k_check_residual(int64_t n, const double
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
-      m0 = fmax(m0, r0[w]);
-      m1 = fmax(m1, r1[w]);
+      m0 = nan_max(m0, r0[w]);
+      m1 = nan_max(m1, r1[w]);
     }
-    out[0] = fmax(m0, r0[0]);
-    out[1] = fmax(m1, r1[0]);
+    out[0] = m0;
+    out[1] = m1;
   }
 }
 
