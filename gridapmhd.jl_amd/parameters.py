@@ -1,7 +1,8 @@
 """Parameter schema of ``GridapMHD.main`` restricted to the hot path.
 
 Mirrors src/parameters.jl: ``add_default_params`` (:85-137), the
-mandatory/optional/unused-key checks (:3-76), ``params_fluid`` (:693-723),
+mandatory/optional/unused-key checks (:3-76, here one ``KeySchema`` per
+dict, with the same error messages), ``params_fluid`` (:693-723),
 ``params_bcs`` and its sub-dicts (:777-947), ``params_fespaces`` (:350-393)
 and ``default_solver_params`` (:219-287).  Keys keep the reference's names
 (``"α"``, ``"ζᵤ"``, ``"thin_wall"``, ``"τ"``...).  Inputs outside the
@@ -20,45 +21,41 @@ class ParamsError(ValueError):
     pass
 
 
-def _check_mandatory(params, mandatory, p):
-    if not isinstance(params, dict):
-        raise ParamsError(f"The params{p} has to be a dict")
-    for key, req in mandatory.items():
-        if req and key not in params:
-            raise ParamsError(f"Key :{key} is a mandatory key in params{p}, but it is not provided.")
+REQUIRED = object()   # the value of a mandatory key in a KeySchema
 
 
-def _add_optional(sub, mandatory, optional, p):
-    out = dict(sub)
-    for key, req in mandatory.items():
-        if not req and key not in optional:
-            raise ParamsError("Internal error")
-    for key, val in optional.items():
-        out.setdefault(key, val)
-    return out
+class KeySchema:
+    """The keys one parameter dict may hold, each with its default value or
+    ``REQUIRED``.  ``validate`` returns a copy with the defaults filled in and
+    raises ParamsError -- with the reference's messages, which callers match
+    on -- for a value that is not a dict, a missing mandatory key and, unless
+    params[:check_valid] is false, a key the schema does not know."""
 
+    def __init__(self, spec: Dict[str, Any]):
+        self.spec = dict(spec)
 
-def _check_unused(sub, mandatory, params, p):
-    if params.get("check_valid", True):
-        for key in sub:
-            if key not in mandatory:
-                raise ParamsError(f"Key :{key} is not a valid key in params{p}. "
+    def validate(self, sub, params=None, where=""):
+        if not isinstance(sub, dict):
+            raise ParamsError(f"The params{where} has to be a dict")
+        missing = [k for k, d in self.spec.items() if d is REQUIRED and k not in sub]
+        if missing:
+            raise ParamsError(f"Key :{missing[0]} is a mandatory key in params{where}, but it is not provided.")
+        out = {k: d for k, d in self.spec.items() if d is not REQUIRED}
+        out.update(sub)
+        if (out if params is None else params).get("check_valid", True):
+            unknown = [k for k in out if k not in self.spec]
+            if unknown:
+                raise ParamsError(f"Key :{unknown[0]} is not a valid key in params{where}. "
                                   f"Set params[:check_valid] = false to ignore invalid keys.")
+        return out
 
-
-def _check_and_add(sub, mandatory, optional, params, p):
-    _check_mandatory(sub, mandatory, p)
-    out = _add_optional(sub, mandatory, optional, p)
-    _check_unused(out, mandatory, params, p)
-    return out
-
-
-def _check_and_add_weak(t, mandatory, optional, params, p):
-    if isinstance(t, dict):
-        return [_check_and_add(t, mandatory, optional, params, p)]
-    if isinstance(t, (list, tuple)):
-        return [_check_and_add(ti, mandatory, optional, params, f"{p}[{i}]") for i, ti in enumerate(t)]
-    raise ParamsError(f"The value params{p} has to be a dict or a list of dicts.")
+    def validate_each(self, value, params, where):
+        """A dict or a list of dicts (the weak-BC entries), as a list."""
+        if isinstance(value, dict):
+            return [self.validate(value, params, where)]
+        if isinstance(value, (list, tuple)):
+            return [self.validate(v, params, f"{where}[{i}]") for i, v in enumerate(value)]
+        raise ParamsError(f"The value params{where} has to be a dict or a list of dicts.")
 
 
 # ---------------------------------------------------------------------------
@@ -108,12 +105,9 @@ def params_solver(params):
 
 def params_fespaces(params):
     fe = params.get("fespaces") or {}
-    mandatory = {"order_u": False, "order_j": False, "p_constraint": False, "φ_constraint": False,
-                 "rt_scaling": False, "fluid_disc": False, "current_disc": False}
-    order_u = fe.get("order_u", 2)
-    optional = {"order_u": 2, "order_j": order_u, "p_constraint": None, "φ_constraint": None,
-                "rt_scaling": None, "fluid_disc": "Qk_dPkm1", "current_disc": "RT"}
-    fe = _check_and_add(fe, mandatory, optional, params, "[:fespaces]")
+    order_u = fe.get("order_u", 2) if isinstance(fe, dict) else 2
+    fe = KeySchema({"order_u": 2, "order_j": order_u, "p_constraint": None, "φ_constraint": None, "rt_scaling": None,
+                    "fluid_disc": "Qk_dPkm1", "current_disc": "RT"}).validate(fe, params, "[:fespaces]")
     if fe["order_u"] != 2 or fe["order_j"] != 2:
         raise NotImplementedError("the GPU kernels implement order_u = order_j = 2 (Q2 / RT1)")
     if fe["fluid_disc"] != "Qk_dPkm1" or fe["current_disc"] != "RT":
@@ -129,11 +123,9 @@ def params_fespaces(params):
 
 
 def params_fluid(params):
-    mandatory = {"domain": True, "α": True, "β": True, "γ": True, "B": True, "f": False, "σ": False, "ζᵤ": False,
-                 "ζⱼ": False, "g": False, "divg": False, "convection": False, "μ": False}
-    optional = {"σ": 1.0, "f": (0.0, 0.0, 0.0), "ζᵤ": 0.0, "ζⱼ": 0.0, "g": (0.0, 0.0, 0.0), "divg": 0.0,
-                "convection": "newton", "μ": 2 * 3}
-    fluid = _check_and_add(params["fluid"], mandatory, optional, params, "[:fluid]")
+    fluid = KeySchema({"domain": REQUIRED, "α": REQUIRED, "β": REQUIRED, "γ": REQUIRED, "B": REQUIRED,
+                       "f": (0.0, 0.0, 0.0), "σ": 1.0, "ζᵤ": 0.0, "ζⱼ": 0.0, "g": (0.0, 0.0, 0.0), "divg": 0.0,
+                       "convection": "newton", "μ": 2 * 3}).validate(params["fluid"], params, "[:fluid]")
     if fluid["convection"] not in ("none", "newton", "picard"):
         raise ParamsError("fluid[:convection] must be one of none, newton, picard")
     dom = fluid["domain"]
@@ -149,12 +141,13 @@ def params_fluid(params):
 
 
 def params_bcs(params):
-    mandatory = {"u": True, "j": True, "φ": False, "t": False, "thin_wall": False, "stabilization": False}
-    optional = {"j": [], "φ": [], "t": [], "thin_wall": [], "stabilization": []}
-    bcs = _check_and_add(params["bcs"], mandatory, optional, params, "[:bcs]")
-    bcs["u"] = _check_and_add(bcs["u"], {"tags": True, "values": False}, {"values": None}, params, "[:bcs][:u]")
+    # the H1-HDiv formulation makes :j mandatory (src/parameters.jl:778-786)
+    bcs = KeySchema({"u": REQUIRED, "j": REQUIRED, "φ": [], "t": [], "thin_wall": [],
+                     "stabilization": []}).validate(params["bcs"], params, "[:bcs]")
+    dirichlet = KeySchema({"tags": REQUIRED, "values": None})
+    bcs["u"] = dirichlet.validate(bcs["u"], params, "[:bcs][:u]")
     if bcs["j"] != []:
-        bcs["j"] = _check_and_add(bcs["j"], {"tags": True, "values": False}, {"values": None}, params, "[:bcs][:j]")
+        bcs["j"] = dirichlet.validate(bcs["j"], params, "[:bcs][:j]")
     else:
         bcs["j"] = {"tags": [], "values": None}
     if bcs["φ"] != []:
@@ -164,28 +157,23 @@ def params_bcs(params):
     if bcs["stabilization"] != []:
         # src/parameters.jl:968-977: the domain defaults to the fluid's
         fluid_dom = params["fluid"]["domain"]
-        bcs["stabilization"] = _check_and_add_weak(bcs["stabilization"], {"domain": False, "μ": True},
-                                                   {"domain": fluid_dom}, params, "[:bcs][:stabilization]")
+        bcs["stabilization"] = KeySchema({"domain": fluid_dom, "μ": REQUIRED}).validate_each(
+            bcs["stabilization"], params, "[:bcs][:stabilization]")
         for st in bcs["stabilization"]:
             if st["domain"] != fluid_dom:
                 raise NotImplementedError("skeleton stabilisation on a domain other than the fluid's")
     if bcs["thin_wall"] != []:
-        bcs["thin_wall"] = _check_and_add_weak(bcs["thin_wall"], {"domain": True, "cw": True, "τ": True, "jw": False},
-                                               {"jw": 0.0}, params, "[:bcs][:thin_wall]")
+        bcs["thin_wall"] = KeySchema({"domain": REQUIRED, "cw": REQUIRED, "τ": REQUIRED, "jw": 0.0}).validate_each(
+            bcs["thin_wall"], params, "[:bcs][:thin_wall]")
     return bcs
 
 
 def add_default_params(_params: Dict[str, Any]) -> Dict[str, Any]:
     """``add_default_params`` (src/parameters.jl:85-137)."""
-    mandatory = {"ptimer": False, "debug": False, "solve": False, "res_assemble": False, "jac_assemble": False,
-                 "model": True, "fluid": True, "solid": False, "bcs": True, "fespaces": False, "solver": True,
-                 "multigrid": False, "check_valid": False, "transient": False, "continuation": False, "x0": False}
-    _check_mandatory(_params, mandatory, "")
-    optional = {"ptimer": None, "debug": False, "solve": True, "res_assemble": False, "jac_assemble": False,
-                "solid": None, "fespaces": None, "multigrid": None, "check_valid": True, "transient": None,
-                "continuation": None, "x0": "zero"}
-    params = _add_optional(_params, mandatory, optional, "")
-    _check_unused(params, mandatory, params, "")
+    params = KeySchema({"ptimer": None, "debug": False, "solve": True, "res_assemble": False, "jac_assemble": False,
+                        "model": REQUIRED, "fluid": REQUIRED, "solid": None, "bcs": REQUIRED, "fespaces": None,
+                        "solver": REQUIRED, "multigrid": None, "check_valid": True, "transient": None,
+                        "continuation": None, "x0": "zero"}).validate(_params)
     for k in ("solid", "multigrid", "transient", "continuation"):
         if params[k] is not None:
             raise NotImplementedError(f"params[:{k}] is outside the stationary H1-HDiv hot path")
