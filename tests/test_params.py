@@ -82,3 +82,10 @@ def test_main_debug_mode_builds_spaces_only():
     sp = xh.spaces
     assert sp.style == "badia2024" and xh.free_values.size == sp.ndofs
     assert "fe_spaces" in out["timings"]
+
+
+def test_hunt_debug_mode_reports_sizes_only():
+    """hunt(debug=true) runs main's debug path (spaces and a random state, no
+    operator): the dof counts are reported, the solution norms are not."""
+    info, xh = g.hunt(nc=(3, 3), B=(0.0, 10.0, 0.0), BL_adapted=False, debug=True, μ=0.2)
+    assert info["ndofs"] == xh.spaces.ndofs and "uh_l2" not in info
