@@ -25,10 +25,12 @@ oracle/cpu_baseline.py + cpu_step.c price one Newton step on the host CPU
 (timing only, for the benchmark's CPU baseline).
 
 The direct solve uses scipy's SuperLU.  A singular Jacobian (constant pressure
-mode of a fully Dirichlet velocity; constant potential when every wall is
-insulating) is made regular by pinning one dof of each null field; the pinned
-equation is implied by the others, so every other unknown is unchanged and the
-pinned field is compared modulo its constant.
+mode of a fully Dirichlet velocity; constant potential when j.n is prescribed
+on the whole boundary) is made regular by pinning one dof of each null field
+(``null_fields`` tells which); the pinned equation is implied by the others,
+so every other unknown is unchanged and the pinned field is compared modulo
+its constant.  A pin on a field without a null mode drops an equation that is
+not implied, and Newton stalls at its residual.
 """
 from __future__ import annotations
 
