@@ -268,7 +268,7 @@ def main():
                          "fine-level smoother in FP64 and FP32), no solves: the run the PMC passes profile")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding).",
+        print("bench.py measures one GPU: libmhd has no multi-GPU path (the solver runs on one device).",
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -3,8 +3,8 @@
 Every test drives libmhd through its C ABI (ctypes) and compares with the
 oracle on the same mesh, dof tables, parameters and state.  Tolerances:
 residual entries 1e-12 relative to the residual norm, Jacobian entries 1e-12
-relative to the largest entry of their row (the north-star bar for matrix
-values), pattern bit-exact.
+relative to the largest entry of their row (the bar for matrix values),
+pattern bit-exact.
 """
 import sys
 
